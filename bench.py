@@ -1,0 +1,216 @@
+"""Benchmark: single-key group-by throughput on one MI355X.
+
+Workload (the strategy sweep the headline metric is quoted on): 2^28 rows,
+int32 keys with distinct count D in {2^4, 2^10, 2^16, 2^22, 2^26}, drawn
+uniform (exact counts, shuffled) or Zipf s in {0.5, 1.0, 1.5}; an int64 value
+uniform in [-2^31, 2^31) and a float32 value uniform in [0, 1); aggregates
+COUNT, SUM/MIN/MAX(int64), SUM(float32).  20 cases.  Inputs are generated on
+the device and stay resident in HBM; each case passes
+hint.groups_upper_bound = D as the reference's benchmark does
+(benchmarks/groupby/groupby_nvbench.cu:57-59).
+
+One step = one group-by of every case.  value = rows / second over the whole
+sweep (Grows/s = N / t / 1e9, the reference's Elem/s).  Algorithmic bytes per
+case = 16 N + 40 G (each input column read once, each output written once).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--strategy hash]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "group-by Grows/s @256M rows, %HBM peak; learned-vs-fixed dispatch speedup"
+DS = [1 << 4, 1 << 10, 1 << 16, 1 << 22, 1 << 26]
+DISTS = [("uniform", 0.0), ("zipf", 0.5), ("zipf", 1.0), ("zipf", 1.5)]
+SEED = 42
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--rows", type=int, default=1 << 28)
+    p.add_argument("--strategy", default="hash", choices=["auto", "hash", "sort", "partition"])
+    p.add_argument("--cases", default="all", help="'all' or comma list like 16:uniform,1024:zipf1.0")
+    p.add_argument("--cpu-sample-rows", type=int, default=1 << 22)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def case_list(spec):
+    cases = [(d, name, s) for d in DS for (name, s) in DISTS]
+    if spec == "all":
+        return cases
+    want = []
+    for tok in spec.split(","):
+        d, dist = tok.split(":")
+        s = 0.0 if dist == "uniform" else float(dist.replace("zipf", ""))
+        want.append((int(d), "uniform" if s == 0.0 else "zipf", s))
+    return want
+
+
+def case_name(c):
+    d, name, s = c
+    return f"D=2^{d.bit_length() - 1} {name}" + ("" if name == "uniform" else f" s={s}")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from eiger_amd import gamax as G
+
+    A = G.aggregation_id
+    n = args.rows
+    cases = case_list(args.cases)
+    fn = {"auto": G.groupby, "hash": G.hash_groupby, "sort": G.sort_groupby,
+          "partition": G.partition_groupby}[args.strategy]
+
+    # ---- inputs resident in HBM -------------------------------------------------
+    t0 = time.time()
+    v64 = G.gen_values(n, G.val_dist.UNIFORM_I64, G.type_id.INT64, seed=SEED, stream_id=2)
+    vf = G.gen_values(n, G.val_dist.UNIFORM_F32, G.type_id.FLOAT32, seed=SEED, stream_id=3)
+    keys = []
+    for c in cases:
+        d, name, s = c
+        if name == "uniform":
+            keys.append(G.gen_keys_uniform(n, d, seed=SEED, t=G.type_id.INT32))
+        else:
+            keys.append(G.gen_keys_zipf(n, d, s, seed=SEED, t=G.type_id.INT32))
+    log(f"generated {len(cases)} cases x {n} rows in {time.time() - t0:.1f}s")
+    reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MIN), (0, A.MAX), (1, A.SUM)]
+
+    def run_case(i):
+        h = G.groupby_hint(groups_upper_bound=cases[i][0])
+        t = fn([keys[i]], [v64, vf], reqs, h)
+        g = t.num_rows()
+        t.free()
+        return g
+
+    # ---- copy roofline probe (also the PMC calibration kernel) -------------------
+    copy_ms = G.probe_copy(1 << 32, iters=5)
+    copy_gbs = 2 * (1 << 32) / (copy_ms * 1e-3) / 1e9
+    log(f"copy probe: {copy_gbs:.0f} GB/s (read+write of 4 GiB)")
+
+    # ---- warmup + timed steps ----------------------------------------------------
+    groups = [0] * len(cases)
+    for w in range(args.warmup):
+        for i in range(len(cases)):
+            groups[i] = run_case(i)
+    G.kernel_timing_reset()
+    G.kernel_timing(True)
+    per_case = [0.0] * len(cases)
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        for i in range(len(cases)):
+            a = time.perf_counter()
+            groups[i] = run_case(i)
+            per_case[i] += time.perf_counter() - a
+        log(f"step {k + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t_start
+    G.kernel_timing(False)
+
+    rows = n * len(cases) * args.steps
+    value = rows / elapsed / 1e9
+    alg_bytes = sum(16 * n + 40 * g for g in groups) * args.steps
+    sweep = []
+    for i, c in enumerate(cases):
+        t = per_case[i] / args.steps
+        b = 16 * n + 40 * groups[i]
+        sweep.append({"case": case_name(c), "groups": groups[i], "ms": round(t * 1e3, 3),
+                      "grows_per_s": round(n / t / 1e9, 2), "hbm_frac": round(b / t / PEAK_HBM, 4)})
+        log(f"  {case_name(c):24s} G={groups[i]:>9d}  {t * 1e3:8.3f} ms  {n / t / 1e9:7.2f} Grows/s  "
+            f"{b / t / PEAK_HBM * 100:5.1f}% HBM")
+
+    # ---- dominant kernel roofline --------------------------------------------------
+    kernels = {}
+    for name in ("hash_agg", "hash_compact", "hll", "sort_pass", "partition_pass", "partition_agg",
+                 "segreduce"):
+        cnt, ms = G.kernel_time(name)
+        if cnt:
+            kernels[name] = (cnt, ms)
+    roofline = None
+    if kernels:
+        dom = max(kernels, key=lambda k: kernels[k][1])
+        cnt, ms = kernels[dom]
+        avg_s = ms / cnt * 1e-3
+        per_launch = 16 * n  # algorithmic bytes per launch: 16 B/row x rows (inputs read once)
+        achieved = per_launch / avg_s / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9,
+                    "unit": "GB/s", "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": None,
+                    "launches": cnt, "avg_ms": round(ms / cnt, 4),
+                    "bytes_per_launch": per_launch, "copy_probe_gbs": round(copy_gbs, 1)}
+        log(f"dominant kernel {dom}: {cnt} launches, avg {ms / cnt:.3f} ms, {achieved:.0f} GB/s algorithmic")
+        for k, (c, m) in kernels.items():
+            log(f"  kernel {k:16s} launches={c:4d} total={m:9.3f} ms")
+
+    # ---- CPU baseline (oracle, OpenMP) on a bounded sample -------------------------
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(cases, n, args.cpu_sample_rows)
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Grows/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic (counter-based Philox generator, seed 42, resident in HBM)",
+        "config": {"workload": "single-query strategy sweep: 2^28-row int32-key group-by, D in "
+                               "{2^4,2^10,2^16,2^22,2^26} x {uniform, zipf 0.5/1.0/1.5}; COUNT, "
+                               "SUM/MIN/MAX(int64), SUM(float32)",
+                   "rows": n, "cases": len(cases), "strategy": args.strategy,
+                   "hbm_frac_algorithmic": round(alg_bytes / elapsed / PEAK_HBM, 4)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "sweep": sweep,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(cases, n, sample):
+    """The oracle's OpenMP group-by (hash partition + per-partition tables) on
+    the first `sample` rows of every case, same aggregates."""
+    import numpy as np
+
+    import oracle as O
+
+    try:
+        threads = min(16, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    sample = min(sample, n)
+    v64 = O.gen_values(sample, O.VAL_UNIFORM_I64, np.int64, seed=SEED, stream_id=2)
+    vf = O.gen_values(sample, O.VAL_UNIFORM_F32, np.float32, seed=SEED, stream_id=3)
+    reqs = [(-1, O.COUNT_ALL), (0, O.SUM), (0, O.MIN), (0, O.MAX), (1, O.SUM)]
+    total = 0.0
+    for d, name, s in cases:
+        if name == "uniform":
+            k = O.gen_keys_uniform(n, d, seed=SEED, begin=0, count=sample)
+        else:
+            k = O.gen_keys_zipf(sample, O.zipf_cdf(d, s), seed=SEED)
+        a = time.perf_counter()
+        O.groupby(k, [v64, vf], reqs, parallel=True, nthreads=threads)
+        total += time.perf_counter() - a
+    rows = sample * len(cases)
+    log(f"cpu baseline: {rows} rows in {total:.2f}s on {threads} threads")
+    return {"value": round(rows / total / 1e9, 4), "unit": "Grows/s", "cores": threads, "kind": "port",
+            "sample": f"first {sample} rows of each of the {len(cases)} cases (same generator, same 5 "
+                      f"aggregates), oracle OpenMP hash-partition group-by"}
+
+
+if __name__ == "__main__":
+    main()

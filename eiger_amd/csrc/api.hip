@@ -1,0 +1,568 @@
+// api.hip -- the extern "C" boundary of libeiger_groupby.so.
+//
+// Mirrors gamax::groupby::groupby (operators/groupby/groupby.cu:121-465):
+// request rewriting (AVG -> SUM + shared count, COUNT_ALL -> the shared count,
+// hint.agg_count -> trailing count), strategy selection, and result assembly
+// [key, one column per request in order, count].  The reference materialises
+// an int64 "ones" column and sums it for counts (groupby.cu:202-231); here the
+// count is a state of its own, so no extra column is written or read.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "eg_internal.hpp"
+
+namespace eg {
+const char* last_error_cstr();
+int timing_enable(int on);
+int timing_reset();
+void timing_query(const char* name, int64_t* launches, double* total_ms);
+void zipf_cdf(double* cdf, int64_t distinct, double s);
+Result run_sort(const Plan& plan, const eg_groupby_hint& hint, bool gfur, hipStream_t s);
+Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream_t s);
+int64_t ndv_approx(const void* data, int32_t type, int64_t n, hipStream_t s);
+void scalar_aggregate(const std::vector<ViewIn>& cols, const eg_agg_request* reqs, int64_t nreqs,
+                      hipStream_t s, Result& res);
+
+// ---- plan ------------------------------------------------------------------------
+namespace {
+struct PlanBuilder {
+    Plan p;
+    std::map<int64_t, int> vcol_of;  // request column index -> AggSpec col
+
+    int vcol(const ViewIn& v, int64_t idx)
+    {
+        auto it = vcol_of.find(idx);
+        if (it != vcol_of.end()) return it->second;
+        if (p.spec.nvcols >= kMaxVCols) raise(EG_ERR_UNSUPPORTED, "too many distinct value columns");
+        int c = p.spec.nvcols++;
+        p.spec.cols[c].data = v.data;
+        p.spec.cols[c].type = v.type;
+        p.spec.cols[c].bytes = type_size(v.type);
+        p.spec.cols[c].conv = v.type == EG_INT32 ? CONV_SEXT32 : v.type == EG_INT16 ? CONV_SEXT16
+                            : v.type == EG_INT8 ? CONV_SEXT8 : v.type == EG_FLOAT32 ? CONV_F32 : CONV_RAW;
+        vcol_of[idx] = c;
+        return c;
+    }
+
+    int op(int32_t code, int col)
+    {
+        for (int a = 0; a < p.spec.nops; ++a)
+            if (p.spec.ops[a].code == code && p.spec.ops[a].col == col) return a;
+        if (p.spec.nops >= kMaxOps) raise(EG_ERR_UNSUPPORTED, "too many aggregation states");
+        int a = p.spec.nops++;
+        p.spec.ops[a].code = code;
+        p.spec.ops[a].col = col;
+        return a;
+    }
+};
+}  // namespace
+
+// Operand masks of each op (see OpDesc) and the class order the kernels
+// expect; output columns are re-pointed at the moved ops.
+void order_ops_by_class(Plan& p)
+{
+    const uint64_t S = 0x8000000000000000ull;
+    AggSpec& sp = p.spec;
+    std::vector<int> order;
+    for (int cls = 0; cls < 4; ++cls) {
+        sp.ncls[cls] = 0;
+        for (int a = 0; a < sp.nops; ++a)
+            if (op_class(sp.ops[a].code) == cls) {
+                order.push_back(a);
+                sp.ncls[cls]++;
+            }
+    }
+    OpDesc old[kMaxOps];
+    std::vector<int> where(sp.nops);
+    for (int a = 0; a < sp.nops; ++a) old[a] = sp.ops[a];
+    for (int i = 0; i < sp.nops; ++i) {
+        OpDesc od = old[order[i]];
+        uint32_t xf = XF_NONE;
+        switch (od.code) {
+        case OPC_MIN_U: od.xmask = ~0ull; od.fmask = 0; xf = XF_MINU; break;
+        case OPC_MAX_I: od.xmask = S; od.fmask = 0; xf = XF_MAXI; break;
+        case OPC_MIN_I: od.xmask = ~S; od.fmask = 0; xf = XF_MINI; break;
+        case OPC_MAX_F: od.xmask = S; od.fmask = ~S; xf = XF_MAXF; break;
+        case OPC_MIN_F: od.xmask = ~S; od.fmask = ~S; xf = XF_MINF; break;
+        default: od.xmask = 0; od.fmask = 0; break;
+        }
+        od.packed = xf | (uint32_t)(od.col + 1) << 4 | (uint32_t)op_class(od.code) << 8;
+        sp.ops[i] = od;
+        where[order[i]] = i;
+    }
+    for (auto& o : p.outs) {
+        if (o.op >= 0) o.op = where[o.op];
+        if (o.op2 >= 0) o.op2 = where[o.op2];
+    }
+}
+
+Plan make_plan(const ViewIn* key, const std::vector<ViewIn>& vals, const eg_agg_request* reqs,
+               int64_t nreqs, bool agg_count, bool rewrite_avg)
+{
+    if (nreqs <= 0 || !reqs) raise(EG_ERR_INVALID, "No aggregation function is specified");
+    PlanBuilder b;
+    if (key) {
+        if (!(key->type == EG_INT32 || key->type == EG_INT64 || key->type == EG_UINT32 || key->type == EG_UINT64))
+            raise(EG_ERR_UNSUPPORTED, "group-by keys must be 4- or 8-byte integers");
+        if (key->size < 0) raise(EG_ERR_INVALID, "negative key column size");
+        if (key->size > 0 && !key->data) raise(EG_ERR_INVALID, "null key column data");
+        b.p.key.data = key->data;
+        b.p.key.bytes = type_size(key->type);
+        b.p.key.is_signed = is_signed_int(key->type) ? 1 : 0;
+        b.p.key_type = key->type;
+        b.p.rows = key->size;
+    } else {
+        // scalar aggregate: no key; rows = the first column's (table_view::num_rows)
+        b.p.key.bytes = 8;
+        b.p.key_type = EG_INT64;
+        b.p.rows = vals.empty() ? 0 : vals[0].size;
+    }
+
+    bool needs_count = agg_count;
+    for (int64_t r = 0; r < nreqs; ++r) {
+        int k = reqs[r].agg_kind;
+        if (k == EG_AGG_AVG && !rewrite_avg)
+            raise(EG_ERR_UNSUPPORTED, "AVG is only rewritten by the top-level groupby");
+        if (k == EG_AGG_AVG || k == EG_AGG_COUNT_ALL) needs_count = true;
+    }
+    int count_op = needs_count ? b.op(OPC_COUNT, -1) : -1;
+
+    for (int64_t r = 0; r < nreqs; ++r) {
+        const int k = reqs[r].agg_kind;
+        if (k == EG_AGG_COUNT_ALL) {
+            b.p.outs.push_back(OutCol{OUT_U64, EG_INT64, count_op, -1});
+            continue;
+        }
+        if (k < EG_AGG_SUM || k > EG_AGG_ANY) raise(EG_ERR_UNSUPPORTED, "Unsupported aggregation function");
+        const int64_t ci = reqs[r].column_index;
+        if (ci < 0 || ci >= (int64_t)vals.size()) raise(EG_ERR_INVALID, "aggregation column index out of range");
+        const ViewIn& v = vals[(size_t)ci];
+        if (!(is_int(v.type) || is_float(v.type))) raise(EG_ERR_UNSUPPORTED, "unsupported value column type");
+        if (v.size != b.p.rows) raise(EG_ERR_INVALID, "value column size differs from the key column");
+        if (v.size > 0 && !v.data) raise(EG_ERR_INVALID, "null value column data");
+        const int col = b.vcol(v, ci);
+        const bool sg = is_signed_int(v.type), fl = is_float(v.type);
+        const int32_t ot = agg_output_type(k, v.type);
+        switch (k) {
+        case EG_AGG_SUM: {
+            int a = b.op(fl ? OPC_SUM_F : (sg ? OPC_SUM_I : OPC_SUM_U), col);
+            b.p.outs.push_back(OutCol{fl ? (v.type == EG_FLOAT32 ? OUT_F32 : OUT_F64) : OUT_U64, ot, a, -1});
+            break;
+        }
+        case EG_AGG_AVG: {
+            int a = b.op(fl ? OPC_SUM_F : (sg ? OPC_SUM_I : OPC_SUM_U), col);
+            b.p.outs.push_back(OutCol{fl ? OUT_AVG_F : (sg ? OUT_AVG_I : OUT_AVG_U), EG_FLOAT64, a, count_op});
+            break;
+        }
+        case EG_AGG_MAX: {
+            int a = b.op(fl ? OPC_MAX_F : (sg ? OPC_MAX_I : OPC_MAX_U), col);
+            b.p.outs.push_back(OutCol{fl ? OUT_MAX_F : (sg ? OUT_MAX_I : OUT_MAX_U), ot, a, -1});
+            break;
+        }
+        case EG_AGG_MIN: {
+            int a = b.op(fl ? OPC_MIN_F : (sg ? OPC_MIN_I : OPC_MIN_U), col);
+            b.p.outs.push_back(OutCol{fl ? OUT_MIN_F : (sg ? OUT_MIN_I : OUT_MIN_U), ot, a, -1});
+            break;
+        }
+        default: {
+            int a = b.op(OPC_ANY, col);
+            b.p.outs.push_back(OutCol{OUT_ANY, ot, a, -1});
+            break;
+        }
+        }
+    }
+    if (agg_count) b.p.outs.push_back(OutCol{OUT_U64, EG_INT64, count_op, -1});
+    if ((int)b.p.outs.size() > kMaxOuts) raise(EG_ERR_UNSUPPORTED, "too many output columns");
+    order_ops_by_class(b.p);
+    return b.p;
+}
+
+void alloc_outputs(const Plan& plan, int64_t groups, hipStream_t s, Result& res, OutSpec& os)
+{
+    res.num_groups = groups;
+    res.cols.clear();
+    res.types.clear();
+    std::memset(&os, 0, sizeof os);
+    const size_t rows = (size_t)std::max<int64_t>(groups, 0);
+    res.cols.emplace_back(rows * plan.key.bytes, s);
+    res.types.push_back(plan.key_type);
+    os.key_data = res.cols.back().get();
+    os.key_bytes = plan.key.bytes;
+    os.nouts = (int32_t)plan.outs.size();
+    for (size_t j = 0; j < plan.outs.size(); ++j) {
+        const OutCol& o = plan.outs[j];
+        const int bytes = type_size(o.type);
+        res.cols.emplace_back(rows * bytes, s);
+        res.types.push_back(o.type);
+        OutDesc& d = os.outs[j];
+        d.data = res.cols.back().get();
+        d.code = o.code;
+        d.bytes = bytes;
+        d.op = o.op;
+        d.op2 = o.op2;
+        d.is_float = is_float(o.type) ? 1 : 0;
+    }
+}
+
+}  // namespace eg
+
+// ---- extern "C" -----------------------------------------------------------------------
+struct eg_table_s {
+    struct Col {
+        void* data;
+        int64_t size;
+        int32_t type;
+    };
+    std::vector<Col> cols;
+};
+
+namespace {
+using namespace eg;
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+ViewIn view_in(const eg_column_view& v)
+{
+    if (v.offset < 0) raise(EG_ERR_INVALID, "negative column offset");
+    const int bytes = type_size(v.type);
+    if (bytes == 0) raise(EG_ERR_UNSUPPORTED, "unsupported column type");
+    ViewIn r;
+    r.data = v.data ? (const char*)v.data + v.offset * bytes : nullptr;
+    r.size = v.size;
+    r.type = v.type;
+    return r;
+}
+
+eg_table to_table(Result& r)
+{
+    auto* t = new eg_table_s;
+    for (size_t j = 0; j < r.cols.size(); ++j)
+        t->cols.push_back(eg_table_s::Col{r.cols[j].release(), r.num_groups, r.types[j]});
+    return t;
+}
+
+int64_t group_bound_of(const Plan& p, const eg_groupby_hint& h, hipStream_t s)
+{
+    if (h.groups_upper_bound > 0 && h.groups_upper_bound < p.rows) return h.groups_upper_bound;
+    if (p.rows <= 4096) return p.rows;
+    // No usable bound from the caller: HyperLogLog estimate with headroom.
+    int64_t est = ndv_approx(p.key.data, p.key_type, p.rows, s);
+    int64_t b = est + est / 4 + 1024;
+    return std::min<int64_t>(b, p.rows);
+}
+
+enum class Entry { TOP, HASH, SORT, SORT_GFUR, PARTITION, PARTITION_GFUR };
+
+int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
+                  const eg_column_view* vals, int64_t num_vals, const eg_agg_request* reqs,
+                  int64_t num_reqs, eg_groupby_hint* hint, void* stream, eg_table* out)
+{
+    return guard([&] {
+        if (!out) raise(EG_ERR_INVALID, "out is null");
+        *out = nullptr;
+        if (num_keys <= 0 || !keys) raise(EG_ERR_INVALID, "Aggregation only is not supported yet");
+        if (num_keys > 1) raise(EG_ERR_UNSUPPORTED, "only single-column group keys are supported");
+        if (num_vals < 0 || (num_vals > 0 && !vals)) raise(EG_ERR_INVALID, "bad value columns");
+        eg_groupby_hint local;
+        eg_hint_init(&local);
+        eg_groupby_hint& h = hint ? *hint : local;
+        hipStream_t s = as_stream(stream);
+
+        ViewIn k = view_in(keys[0]);
+        std::vector<ViewIn> vs;
+        for (int64_t i = 0; i < num_vals; ++i) vs.push_back(view_in(vals[i]));
+        const bool top = entry == Entry::TOP;
+        Plan plan = make_plan(&k, vs, reqs, num_reqs, top && h.agg_count != 0, top);
+
+        int algo;
+        switch (entry) {
+        case Entry::HASH: algo = EG_ALGO_HASH; break;
+        case Entry::SORT: algo = EG_ALGO_SORT; break;
+        case Entry::SORT_GFUR: algo = EG_ALGO_SORT_GFUR; break;
+        case Entry::PARTITION: algo = EG_ALGO_PARTITION; break;
+        case Entry::PARTITION_GFUR: algo = EG_ALGO_PARTITION_GFUR; break;
+        default:
+            if (h.groupby_algo == EG_ALGO_AUTO && plan.rows > 0) {
+                int rc = eg_auto_select_algo(keys, num_keys, &h, stream);
+                if (rc != EG_OK) raise(rc, last_error_cstr());
+            }
+            algo = h.groupby_algo;
+            if (algo == EG_ALGO_AUTO) algo = EG_ALGO_HASH;  // empty input
+            break;
+        }
+        if (algo == EG_ALGO_SORT_OPT) h.optimize_sort = 1;
+
+        Result res;
+        if (plan.rows == 0) {
+            OutSpec os;
+            alloc_outputs(plan, 0, s, res, os);
+        } else {
+            switch (algo) {
+            case EG_ALGO_HASH: res = run_hash(plan, group_bound_of(plan, h, s), s); break;
+            case EG_ALGO_SORT: case EG_ALGO_SORT_OPT: res = run_sort(plan, h, false, s); break;
+            case EG_ALGO_SORT_GFUR: res = run_sort(plan, h, true, s); break;
+            case EG_ALGO_PARTITION: res = run_partition(plan, group_bound_of(plan, h, s), false, s); break;
+            case EG_ALGO_PARTITION_GFUR: res = run_partition(plan, group_bound_of(plan, h, s), true, s); break;
+            default: raise(EG_ERR_UNSUPPORTED, "Unsupported groupby algorithm");
+            }
+        }
+        EG_HIP(hipStreamSynchronize(s));
+        *out = to_table(res);
+    });
+}
+}  // namespace
+
+extern "C" {
+
+const char* eg_last_error(void) { return eg::last_error_cstr(); }
+
+const char* eg_version(void) { return "eiger-mi355x 0.1 (gfx950)"; }
+
+void eg_hint_init(eg_groupby_hint* h)
+{
+    if (!h) return;
+    h->groups_upper_bound = INT64_MAX;
+    h->zero_based_ids = 0;
+    h->optimize_sort = 0;
+    h->groupby_algo = EG_ALGO_AUTO;
+    h->agg_count = 0;
+    h->enable_skf = 0;
+    h->allow_hash_fallback = 0;
+}
+
+int eg_groupby(const eg_column_view* keys, int64_t num_keys, const eg_column_view* vals,
+               int64_t num_vals, const eg_agg_request* reqs, int64_t num_reqs,
+               eg_groupby_hint* hint, void* stream, eg_table* out)
+{
+    return groupby_entry(Entry::TOP, keys, num_keys, vals, num_vals, reqs, num_reqs, hint, stream, out);
+}
+
+int eg_groupby_hash(const eg_column_view* keys, int64_t num_keys, const eg_column_view* vals,
+                    int64_t num_vals, const eg_agg_request* reqs, int64_t num_reqs,
+                    eg_groupby_hint* hint, void* stream, eg_table* out)
+{
+    return groupby_entry(Entry::HASH, keys, num_keys, vals, num_vals, reqs, num_reqs, hint, stream, out);
+}
+
+int eg_groupby_sort(const eg_column_view* keys, int64_t num_keys, const eg_column_view* vals,
+                    int64_t num_vals, const eg_agg_request* reqs, int64_t num_reqs,
+                    eg_groupby_hint* hint, void* stream, eg_table* out)
+{
+    return groupby_entry(Entry::SORT, keys, num_keys, vals, num_vals, reqs, num_reqs, hint, stream, out);
+}
+
+int eg_groupby_sort_gfur(const eg_column_view* keys, int64_t num_keys, const eg_column_view* vals,
+                         int64_t num_vals, const eg_agg_request* reqs, int64_t num_reqs,
+                         eg_groupby_hint* hint, void* stream, eg_table* out)
+{
+    return groupby_entry(Entry::SORT_GFUR, keys, num_keys, vals, num_vals, reqs, num_reqs, hint, stream, out);
+}
+
+int eg_groupby_partition(const eg_column_view* keys, int64_t num_keys, const eg_column_view* vals,
+                         int64_t num_vals, const eg_agg_request* reqs, int64_t num_reqs,
+                         eg_groupby_hint* hint, void* stream, eg_table* out)
+{
+    return groupby_entry(Entry::PARTITION, keys, num_keys, vals, num_vals, reqs, num_reqs, hint, stream, out);
+}
+
+int eg_groupby_partition_gfur(const eg_column_view* keys, int64_t num_keys,
+                              const eg_column_view* vals, int64_t num_vals,
+                              const eg_agg_request* reqs, int64_t num_reqs,
+                              eg_groupby_hint* hint, void* stream, eg_table* out)
+{
+    return groupby_entry(Entry::PARTITION_GFUR, keys, num_keys, vals, num_vals, reqs, num_reqs, hint, stream, out);
+}
+
+int eg_aggregate(const eg_column_view* cols, int64_t num_cols, const eg_agg_request* reqs,
+                 int64_t num_reqs, void* stream, eg_table* out)
+{
+    return guard([&] {
+        if (!out) raise(EG_ERR_INVALID, "out is null");
+        *out = nullptr;
+        if (num_cols < 0 || (num_cols > 0 && !cols)) raise(EG_ERR_INVALID, "bad input columns");
+        if (num_reqs <= 0 || !reqs) raise(EG_ERR_INVALID, "No aggregation function is specified");
+        std::vector<ViewIn> vs;
+        for (int64_t i = 0; i < num_cols; ++i) vs.push_back(view_in(cols[i]));
+        Result res;
+        scalar_aggregate(vs, reqs, num_reqs, as_stream(stream), res);
+        EG_HIP(hipStreamSynchronize(as_stream(stream)));
+        *out = to_table(res);
+    });
+}
+
+int eg_agg_output_type(int32_t agg_kind, int32_t type, int32_t* out_type)
+{
+    return guard([&] {
+        if (!out_type) raise(EG_ERR_INVALID, "out_type is null");
+        int32_t t = eg::agg_output_type(agg_kind, type);
+        if (t == EG_TYPE_EMPTY) raise(EG_ERR_UNSUPPORTED, "Unsupported aggregation function or type");
+        *out_type = t;
+    });
+}
+
+int64_t eg_table_num_columns(eg_table t) { return t ? (int64_t)t->cols.size() : 0; }
+
+int64_t eg_table_num_rows(eg_table t)
+{
+    // table::num_rows (memory/table.hpp:160-162): rows of the first column.
+    return (t && !t->cols.empty()) ? t->cols[0].size : 0;
+}
+
+int eg_table_get_column(eg_table t, int64_t i, eg_column_view* out)
+{
+    return guard([&] {
+        if (!t || !out) raise(EG_ERR_INVALID, "null argument");
+        if (i < 0 || i >= (int64_t)t->cols.size()) raise(EG_ERR_INVALID, "column out of range");
+        const auto& c = t->cols[(size_t)i];
+        out->data = c.data;
+        out->size = c.size;
+        out->type = c.type;
+        out->reserved = 0;
+        out->offset = 0;
+    });
+}
+
+int eg_table_release_column(eg_table t, int64_t i, eg_column_view* out)
+{
+    return guard([&] {
+        if (!t || !out) raise(EG_ERR_INVALID, "null argument");
+        if (i < 0 || i >= (int64_t)t->cols.size()) raise(EG_ERR_INVALID, "column out of range");
+        auto& c = t->cols[(size_t)i];
+        out->data = c.data;
+        out->size = c.size;
+        out->type = c.type;
+        out->reserved = 0;
+        out->offset = 0;
+        c.data = nullptr;
+    });
+}
+
+void eg_table_free(eg_table t)
+{
+    if (!t) return;
+    for (auto& c : t->cols)
+        if (c.data) {
+            try {
+                eg::mm_free(c.data);
+            } catch (...) {
+            }
+        }
+    delete t;
+}
+
+int eg_malloc(void** ptr, int64_t bytes, int32_t clear, void* stream)
+{
+    return guard([&] {
+        if (!ptr || bytes < 0) raise(EG_ERR_INVALID, "bad arguments");
+        *ptr = eg::mm_alloc((size_t)bytes, as_stream(stream));
+        if (clear) {
+            eg::fill_zero(*ptr, (size_t)bytes, as_stream(stream));
+            EG_HIP(hipStreamSynchronize(as_stream(stream)));
+        }
+    });
+}
+
+int eg_free(void* ptr)
+{
+    return guard([&] { eg::mm_free(ptr); });
+}
+
+int eg_memory_usage(int64_t* current_bytes, int64_t* peak_bytes)
+{
+    return guard([&] { eg::mm_usage(current_bytes, peak_bytes); });
+}
+
+int eg_set_memory_mode(int32_t mode, int64_t pool_bytes)
+{
+    return guard([&] { eg::mm_set_mode(mode, pool_bytes); });
+}
+
+int eg_make_column_from_host(const void* host_data, int64_t num_items, int32_t type, void* stream,
+                             eg_column_view* out)
+{
+    return guard([&] {
+        if (!out || num_items < 0 || (num_items > 0 && !host_data)) raise(EG_ERR_INVALID, "bad arguments");
+        const int bytes = type_size(type);
+        if (!bytes) raise(EG_ERR_UNSUPPORTED, "unsupported column type");
+        void* p = eg::mm_alloc((size_t)num_items * bytes, as_stream(stream));
+        if (num_items > 0) {
+            hipError_t e = hipMemcpyAsync(p, host_data, (size_t)num_items * bytes, hipMemcpyHostToDevice,
+                                          as_stream(stream));
+            if (e == hipSuccess) e = hipStreamSynchronize(as_stream(stream));
+            if (e != hipSuccess) {
+                eg::mm_free(p);
+                raise(EG_ERR_DEVICE, std::string("make_column_from_host: ") + hipGetErrorString(e));
+            }
+        }
+        out->data = p;
+        out->size = num_items;
+        out->type = type;
+        out->reserved = 0;
+        out->offset = 0;
+    });
+}
+
+int eg_copy_column_to_host(eg_column_view col, void* host_data, void* stream)
+{
+    return guard([&] {
+        const int bytes = type_size(col.type);
+        if (!bytes) raise(EG_ERR_UNSUPPORTED, "unsupported column type");
+        if (col.size < 0 || col.offset < 0) raise(EG_ERR_INVALID, "bad column view");
+        if (col.size == 0) return;
+        if (!host_data || !col.data) raise(EG_ERR_INVALID, "null pointer");
+        EG_HIP(hipMemcpyAsync(host_data, (const char*)col.data + col.offset * bytes, (size_t)col.size * bytes,
+                              hipMemcpyDeviceToHost, as_stream(stream)));
+        EG_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+int eg_kernel_timing_enable(int32_t on) { return eg::timing_enable(on); }
+
+int eg_kernel_timing_reset(void) { return eg::timing_reset(); }
+
+int eg_kernel_timing_query(const char* name, int64_t* launches, double* total_ms)
+{
+    return guard([&] {
+        if (!name) raise(EG_ERR_INVALID, "name is null");
+        eg::timing_query(name, launches, total_ms);
+    });
+}
+
+int eg_gen_keys_uniform(void* out, int32_t type, int64_t begin, int64_t count, int64_t n,
+                        int64_t distinct, uint64_t seed, void* stream)
+{
+    return guard([&] {
+        eg::gen_keys_uniform(out, type, begin, count, n, distinct, seed, as_stream(stream));
+        EG_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+int eg_zipf_cdf(double* cdf, int64_t distinct, double s)
+{
+    return guard([&] { eg::zipf_cdf(cdf, distinct, s); });
+}
+
+int eg_gen_keys_zipf(void* out, int32_t type, int64_t begin, int64_t count, const double* d_cdf,
+                     int64_t distinct, uint64_t seed, void* stream)
+{
+    return guard([&] {
+        eg::gen_keys_zipf(out, type, begin, count, d_cdf, distinct, seed, as_stream(stream));
+        EG_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+int eg_gen_values(void* out, int32_t type, int32_t dist, int64_t begin, int64_t count,
+                  uint64_t seed, uint32_t stream_id, void* stream)
+{
+    return guard([&] {
+        eg::gen_values(out, type, dist, begin, count, seed, stream_id, as_stream(stream));
+        EG_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+}  // extern "C"

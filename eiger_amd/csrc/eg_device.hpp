@@ -1,0 +1,350 @@
+// eg_device.hpp -- device-side building blocks shared by every group-by kernel:
+// hashing, key and order encodings, typed loads, operand construction, and the
+// emission of one finished group into the output columns.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eg_spec.hpp"
+
+namespace eg {
+namespace dev {
+
+constexpr uint64_t kSign64 = 0x8000000000000000ull;
+constexpr uint32_t kSign32 = 0x80000000u;
+
+// MurmurHash3 finalizers (Austin Appleby, public domain); the reference uses
+// the same mixers for partition ids (operators/groupby/groupby_partition.cu:64-90).
+__device__ __forceinline__ uint32_t fmix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ uint64_t fmix64(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+// bit_mix of groupby_partition.cu:83-90: fmix32 for keys of <= 4 bytes,
+// fmix64 otherwise.
+template <int KB>
+__device__ __forceinline__ uint64_t bit_mix(uint64_t raw)
+{
+    if constexpr (KB <= 4) return (uint64_t)fmix32((uint32_t)raw);
+    else return fmix64(raw);
+}
+
+// ---- key encodings -------------------------------------------------------
+// Table keys are stored encoded so that the all-zero word means "empty":
+//   enc64(k) = widen(k) ^ 2^63   (widen = sign- or zero-extension)
+// For 4-byte keys enc64 is never zero.  For 8-byte keys exactly one key value
+// (INT64_MIN, or 2^63 for uint64) encodes to zero; it lives in a side slot.
+__device__ __forceinline__ uint64_t widen_key(uint64_t raw, int kb, int is_signed)
+{
+    if (kb == 4) return is_signed ? (uint64_t)(int64_t)(int32_t)(uint32_t)raw : (uint64_t)(uint32_t)raw;
+    return raw;
+}
+
+__device__ __forceinline__ uint64_t enc64(uint64_t wide) { return wide ^ kSign64; }
+__device__ __forceinline__ uint64_t dec64(uint64_t enc) { return enc ^ kSign64; }
+
+// ---- order-preserving encodings -----------------------------------------
+__device__ __forceinline__ uint64_t ord_i64(int64_t v) { return (uint64_t)v ^ kSign64; }
+
+__device__ __forceinline__ uint64_t ord_f64(double d)
+{
+    uint64_t b = (uint64_t)__double_as_longlong(d);
+    return (b & kSign64) ? ~b : (b | kSign64);
+}
+
+__device__ __forceinline__ double unord_f64(uint64_t o)
+{
+    uint64_t b = (o & kSign64) ? (o ^ kSign64) : ~o;
+    return __longlong_as_double((long long)b);
+}
+
+// ---- typed loads -----------------------------------------------------------
+// Raw element bits, zero-extended to 64 bits.
+__device__ __forceinline__ uint64_t load_raw(const void* p, int bytes, int64_t i)
+{
+    switch (bytes) {
+    case 1: return ((const uint8_t*)p)[i];
+    case 2: return ((const uint16_t*)p)[i];
+    case 4: return ((const uint32_t*)p)[i];
+    default: return ((const uint64_t*)p)[i];
+    }
+}
+
+// type ids (gamax::type_id numbering)
+enum : int32_t {
+    T_INT8 = 1, T_INT16 = 2, T_INT32 = 3, T_INT64 = 4, T_BOOL8 = 5, T_UINT8 = 6,
+    T_UINT16 = 7, T_UINT32 = 8, T_UINT64 = 9, T_FLOAT32 = 10, T_FLOAT64 = 11
+};
+
+__host__ __device__ __forceinline__ int type_bytes(int32_t type)
+{
+    switch (type) {
+    case T_INT8: case T_UINT8: case T_BOOL8: return 1;
+    case T_INT16: case T_UINT16: return 2;
+    case T_INT32: case T_UINT32: case T_FLOAT32: return 4;
+    default: return 8;
+    }
+}
+
+__device__ __forceinline__ int64_t raw_as_i64(uint64_t raw, int32_t type)
+{
+    switch (type) {
+    case T_INT8: return (int64_t)(int8_t)(uint8_t)raw;
+    case T_INT16: return (int64_t)(int16_t)(uint16_t)raw;
+    case T_INT32: return (int64_t)(int32_t)(uint32_t)raw;
+    default: return (int64_t)raw;
+    }
+}
+
+__device__ __forceinline__ double raw_as_f64(uint64_t raw, int32_t type)
+{
+    switch (type) {
+    case T_FLOAT32: return (double)__uint_as_float((uint32_t)raw);
+    case T_FLOAT64: return __longlong_as_double((long long)raw);
+    case T_INT8: case T_INT16: case T_INT32: case T_INT64: return (double)raw_as_i64(raw, type);
+    default: return (double)raw;
+    }
+}
+
+// Canonical 64-bit value of a column element (ConvCode): integers sign- or
+// zero-extended, float32 widened to double bits, 8-byte types unchanged.
+__device__ __forceinline__ uint64_t canon(uint64_t raw, int32_t conv)
+{
+    switch (conv) {
+    case CONV_SEXT32: return (uint64_t)(int64_t)(int32_t)(uint32_t)raw;
+    case CONV_F32: return (uint64_t)__double_as_longlong((double)__uint_as_float((uint32_t)raw));
+    case CONV_SEXT16: return (uint64_t)(int64_t)(int16_t)(uint16_t)raw;
+    case CONV_SEXT8: return (uint64_t)(int64_t)(int8_t)(uint8_t)raw;
+    default: return raw;
+    }
+}
+
+// Same with the conversion known at compile time.
+template <int CONV>
+__device__ __forceinline__ uint64_t canon_c(uint64_t raw)
+{
+    if constexpr (CONV == CONV_SEXT32) return (uint64_t)(int64_t)(int32_t)(uint32_t)raw;
+    else if constexpr (CONV == CONV_F32) return (uint64_t)__double_as_longlong((double)__uint_as_float((uint32_t)raw));
+    else if constexpr (CONV == CONV_SEXT16) return (uint64_t)(int64_t)(int16_t)(uint16_t)raw;
+    else if constexpr (CONV == CONV_SEXT8) return (uint64_t)(int64_t)(int8_t)(uint8_t)raw;
+    else return raw;
+}
+
+// Operand of op `od` for canonical value v (see OpDesc).
+__device__ __forceinline__ uint64_t operand(const OpDesc& od, uint64_t v)
+{
+    return v ^ od.xmask ^ ((uint64_t)((int64_t)v >> 63) & od.fmask);
+}
+
+// Class of op a from the class-ordered op list.
+__device__ __forceinline__ int32_t class_of(const AggSpec& spec, int a)
+{
+    const int e0 = spec.ncls[0], e1 = e0 + spec.ncls[1], e2 = e1 + spec.ncls[2];
+    return a < e0 ? CLS_ADD_U64 : a < e1 ? CLS_ADD_F64 : a < e2 ? CLS_MAX_U64 : CLS_XCHG;
+}
+
+// ---- state updates ---------------------------------------------------------
+// Merge operand `v` into the 64-bit state at `p` (LDS or global; the compiler
+// picks ds_* or global_* atomics from the address space of the pointer).
+template <typename P>
+__device__ __forceinline__ void apply_atomic(int32_t cls, P* p, uint64_t v)
+{
+    switch (cls) {
+    case CLS_ADD_U64: atomicAdd((unsigned long long*)p, (unsigned long long)v); break;
+    case CLS_ADD_F64: atomicAdd((double*)p, __longlong_as_double((long long)v)); break;
+    case CLS_MAX_U64: atomicMax((unsigned long long*)p, (unsigned long long)v); break;
+    default: atomicExch((unsigned long long*)p, (unsigned long long)v); break;
+    }
+}
+
+// Non-atomic merge (single owner of the state).
+__device__ __forceinline__ uint64_t combine(int32_t cls, uint64_t a, uint64_t b)
+{
+    switch (cls) {
+    case CLS_ADD_U64: return a + b;
+    case CLS_ADD_F64:
+        return (uint64_t)__double_as_longlong(__longlong_as_double((long long)a) +
+                                              __longlong_as_double((long long)b));
+    case CLS_MAX_U64: return a > b ? a : b;
+    default: return b;
+    }
+}
+
+// Merges the row (canonical column values cv) into the states at
+// ptr(a) for every op, one tight loop per atomic class.
+template <int NV, class PtrFn>
+__device__ __forceinline__ void apply_ops(const AggSpec& spec, const uint64_t (&cv)[NV], PtrFn ptr)
+{
+    auto val = [&](int a) -> uint64_t {
+        const OpDesc& od = spec.ops[a];
+        uint64_t v = 1ull;
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+            if (c == od.col) v = cv[c];
+        return operand(od, v);
+    };
+    const int e0 = spec.ncls[0], e1 = e0 + spec.ncls[1], e2 = e1 + spec.ncls[2];
+    int a = 0;
+    for (; a < e0; ++a) atomicAdd((unsigned long long*)ptr(a), (unsigned long long)val(a));
+    for (; a < e1; ++a) atomicAdd((double*)ptr(a), __longlong_as_double((long long)val(a)));
+    for (; a < e2; ++a) atomicMax((unsigned long long*)ptr(a), (unsigned long long)val(a));
+    for (; a < spec.nops; ++a) atomicExch((unsigned long long*)ptr(a), (unsigned long long)val(a));
+}
+
+// Packed operand transform (XForm) on 32-bit halves; kind is wave-uniform.
+__device__ __forceinline__ uint64_t xform(uint32_t kind, uint64_t v)
+{
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    switch (kind) {
+    case XF_MAXI: hi ^= 0x80000000u; break;
+    case XF_MINI: lo = ~lo; hi ^= 0x7fffffffu; break;
+    case XF_MINU: lo = ~lo; hi = ~hi; break;
+    case XF_MAXF: {
+        const uint32_t m = (uint32_t)((int32_t)hi >> 31);
+        lo ^= m;
+        hi ^= (m & 0x7fffffffu) ^ 0x80000000u;
+        break;
+    }
+    case XF_MINF: {
+        const uint32_t m = (uint32_t)((int32_t)hi >> 31);
+        lo ^= ~m;
+        hi ^= ~((m & 0x7fffffffu) ^ 0x80000000u);
+        break;
+    }
+    default: break;
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Hot-loop variant for up to MAXOPS ops: the op loop is unrolled and each
+// op is one packed 32-bit word (stays in a scalar register), so the row costs
+// a column select, the transform and one atomic per op.  State of op a is at
+// base + a * stride.
+template <int NV, int MAXOPS, class P>
+__device__ __forceinline__ void apply_packed(const AggSpec& spec, const uint64_t (&cv)[NV], P* base, int64_t stride)
+{
+#pragma unroll
+    for (int a = 0; a < MAXOPS; ++a) {
+        if (a >= spec.nops) break;
+        const uint32_t pk = spec.ops[a].packed;
+        const uint32_t colp1 = (pk >> 4) & 15u;
+        uint64_t v = 1ull;
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+            if ((uint32_t)(c + 1) == colp1) v = cv[c];
+        v = xform(pk & 15u, v);
+        P* p = base + a * stride;
+        switch ((pk >> 8) & 3u) {
+        case CLS_ADD_U64: atomicAdd((unsigned long long*)p, (unsigned long long)v); break;
+        case CLS_ADD_F64: atomicAdd((double*)p, __longlong_as_double((long long)v)); break;
+        case CLS_MAX_U64: atomicMax((unsigned long long*)p, (unsigned long long)v); break;
+        default: atomicExch((unsigned long long*)p, (unsigned long long)v); break;
+        }
+    }
+}
+
+// Same, merging 64-bit partial states st(a) (already operands).
+template <class PtrFn, class StFn>
+__device__ __forceinline__ void merge_states(const AggSpec& spec, PtrFn ptr, StFn st)
+{
+    const int e0 = spec.ncls[0], e1 = e0 + spec.ncls[1], e2 = e1 + spec.ncls[2];
+    int a = 0;
+    for (; a < e0; ++a) atomicAdd((unsigned long long*)ptr(a), (unsigned long long)st(a));
+    for (; a < e1; ++a) atomicAdd((double*)ptr(a), __longlong_as_double((long long)st(a)));
+    for (; a < e2; ++a) atomicMax((unsigned long long*)ptr(a), (unsigned long long)st(a));
+    for (; a < spec.nops; ++a) atomicExch((unsigned long long*)ptr(a), (unsigned long long)st(a));
+}
+
+// ---- output ----------------------------------------------------------------
+__device__ __forceinline__ void store_bytes(void* p, int bytes, int64_t i, uint64_t bits)
+{
+    switch (bytes) {
+    case 1: ((uint8_t*)p)[i] = (uint8_t)bits; break;
+    case 2: ((uint16_t*)p)[i] = (uint16_t)bits; break;
+    case 4: ((uint32_t*)p)[i] = (uint32_t)bits; break;
+    default: ((uint64_t*)p)[i] = bits; break;
+    }
+}
+
+__device__ __forceinline__ void store_double_as(void* p, int bytes, int64_t i, double d)
+{
+    if (bytes == 4) ((float*)p)[i] = (float)d;
+    else ((double*)p)[i] = d;
+}
+
+// Writes group `g`: its key (from the 64-bit encoding) and every output
+// column.  States of op a are read at st[a * stride].
+template <typename W>
+__device__ __forceinline__ void emit_group(const OutSpec& os, int64_t g, uint64_t key_enc,
+                                           const W* st, int64_t stride)
+{
+    store_bytes(os.key_data, os.key_bytes, g, dec64(key_enc));
+    for (int j = 0; j < os.nouts; ++j) {
+        const OutDesc& o = os.outs[j];
+        uint64_t s = (uint64_t)st[(int64_t)o.op * stride];
+        switch (o.code) {
+        case OUT_U64: store_bytes(o.data, 8, g, s); break;
+        case OUT_F32: ((float*)o.data)[g] = (float)__longlong_as_double((long long)s); break;
+        case OUT_F64: ((double*)o.data)[g] = __longlong_as_double((long long)s); break;
+        case OUT_MAX_I: store_bytes(o.data, o.bytes, g, s ^ kSign64); break;
+        case OUT_MIN_I: store_bytes(o.data, o.bytes, g, (~s) ^ kSign64); break;
+        case OUT_MAX_U: store_bytes(o.data, o.bytes, g, s); break;
+        case OUT_MIN_U: store_bytes(o.data, o.bytes, g, ~s); break;
+        case OUT_MAX_F: store_double_as(o.data, o.bytes, g, unord_f64(s)); break;
+        case OUT_MIN_F: store_double_as(o.data, o.bytes, g, unord_f64(~s)); break;
+        case OUT_ANY:
+            // canonical value: truncation restores integers, floats come back from double
+            if (o.is_float && o.bytes == 4) ((float*)o.data)[g] = (float)__longlong_as_double((long long)s);
+            else store_bytes(o.data, o.bytes, g, s);
+            break;
+        default: {
+            double cnt = (double)(uint64_t)st[(int64_t)o.op2 * stride];
+            double sum = (o.code == OUT_AVG_I) ? (double)(int64_t)s
+                       : (o.code == OUT_AVG_U) ? (double)s
+                       : __longlong_as_double((long long)s);
+            ((double*)o.data)[g] = sum / cnt;
+            break;
+        }
+        }
+    }
+}
+
+// ---- wave helpers ----------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt()
+{
+    int l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Wave-aggregated increment of a device counter: one atomic per wave; returns
+// this lane's slot.  Every lane of the wave must call it.
+__device__ __forceinline__ int64_t wave_claim(unsigned long long* counter, bool take)
+{
+    uint64_t m = __ballot(take);
+    int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned long long base = 0;
+    if (m != 0 && lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    return (int64_t)(base + __popcll(m & lanemask_lt()));
+}
+
+}  // namespace dev
+}  // namespace eg

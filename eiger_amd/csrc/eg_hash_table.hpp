@@ -1,0 +1,75 @@
+// eg_hash_table.hpp -- the HBM aggregation table shared by the hash and
+// partition strategies: open addressing with linear probing over slots of
+// W = 2^k 64-bit words [encoded key | states...], one cache line or less per
+// slot.  Keys move 0 -> key exactly once (compare-and-swap), so a stale read
+// of an empty slot only costs one extra CAS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eg_device.hpp"
+#include "eg_internal.hpp"
+
+namespace eg {
+
+struct GTable {
+    unsigned long long* base;   // (mask+1) * W words
+    unsigned long long* side;   // W words: states of the key whose encoding is 0
+    unsigned long long* meta;   // [0] occupied slots, [1] side used, [2] overflow
+    uint64_t mask;
+    int W;
+};
+
+// Slot of encoded key `enc` (inserted if absent); nullptr when the table is
+// full (meta[2] is then set and the caller retries with a larger table).
+__device__ __forceinline__ unsigned long long* gt_slot(const GTable& gt, uint64_t enc)
+{
+    if (enc == 0) {
+        if (__hip_atomic_load(&gt.meta[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            atomicExch(&gt.meta[1], 1ull);
+        return gt.side;
+    }
+    uint64_t h = dev::fmix64(enc) & gt.mask;
+    for (uint64_t p = 0; p <= gt.mask; ++p) {
+        unsigned long long* slot = gt.base + h * (uint64_t)gt.W;
+        uint64_t cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == enc) return slot;
+        if (cur == 0) {
+            uint64_t prev = atomicCAS(slot, 0ull, (unsigned long long)enc);
+            if (prev == 0) {
+                atomicAdd(&gt.meta[0], 1ull);
+                return slot;
+            }
+            if (prev == enc) return slot;
+        }
+        h = (h + 1) & gt.mask;
+    }
+    atomicExch(&gt.meta[2], 1ull);
+    return nullptr;
+}
+
+int table_words_per_slot(int nops);
+
+// Owns a zeroed HBM table of `slots` (power of two) slots.
+struct HbmTable {
+    DevBuf buf;
+    GTable gt;
+    HbmTable(int64_t slots, int W, hipStream_t s);
+    void read_meta(int64_t* out3, hipStream_t s);   // synchronises s
+    // Appends every occupied slot to the outputs at positions from *counter.
+    void compact(const OutSpec& os, unsigned long long* counter, hipStream_t s);
+};
+
+// LDS table geometry of the hash kernel.
+struct HashLayout {
+    int cr_log2 = -1;     // slots per replica (log2), -1 = no LDS stage
+    int rep_log2 = 0;     // replicas (log2)
+    size_t lds_bytes = 0;
+    int probe_limit = 16;
+    int peel_min = 8;
+};
+
+HashLayout plan_hash_layout(int nops, int key_bytes, int64_t group_bound, const HashTuning& t);
+
+}  // namespace eg

@@ -1,0 +1,594 @@
+// hash_groupby.hip -- hash-based group-by for MI355X (gfx950).
+//
+// Replaces gamax::groupby::hash::groupby (operators/groupby/groupby_hash.cu:302-426).
+// The reference builds dense group ids with a cuco static_set of row indices,
+// then runs a second pass of per-row global atomics (compute_groupid_kernel
+// :48-75, compute_aggregation :77-99).  Here one streaming pass does both:
+//
+//   1. hash_agg_kernel: a persistent grid, one workgroup per CU slot.  Each
+//      workgroup keeps private open-addressing tables in LDS holding the
+//      encoded key and every aggregation state inline.  For small group counts
+//      the table is replicated R times (R up to 64) and lane l uses replica
+//      l mod R, so lanes of a wave never collide on one LDS address; the
+//      replicas are merged in LDS before the flush.  Rows are read with
+//      16-byte vector loads, four consecutive rows per lane, and the next
+//      tile is loaded while the current one is aggregated.  A row that
+//      cannot find a slot within `probe_limit` probes spills to the HBM
+//      table.  Hot keys (many lanes of a wave on one group) are combined by
+//      the wave first (eg_atomics.hpp) so one atomic per state is issued.
+//   2. hash_compact_kernel: scans the HBM table and writes the final key and
+//      output columns at wave-aggregated positions.
+//
+// The HBM table stores slots of W = 2^k 64-bit words [enc key | states...]
+// (AoS, so a random update touches one cache line).  The all-zero word is
+// "empty" for keys and the identity of every state, so the table is cleared
+// with a plain zero fill.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "eg_atomics.hpp"
+#include "eg_device.hpp"
+#include "eg_hash_table.hpp"
+#include "eg_internal.hpp"
+
+namespace eg {
+namespace {
+
+constexpr int kRows = 4;      // consecutive rows per lane per tile
+
+template <int KB>
+struct LdsKey {
+    using T = typename std::conditional<KB == 4, uint32_t, unsigned long long>::type;
+};
+
+// Four consecutive elements (row r multiple of 4, 16-byte aligned column).
+template <int W>
+__device__ __forceinline__ void vload4(const void* p, int64_t r, uint64_t (&o)[kRows])
+{
+    if constexpr (W == 4) {
+        uint4 v = *(const uint4*)((const uint32_t*)p + r);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    } else {
+        const uint4* q = (const uint4*)((const uint64_t*)p + r);
+        uint4 a = q[0], b = q[1];
+        o[0] = a.x | ((uint64_t)a.y << 32); o[1] = a.z | ((uint64_t)a.w << 32);
+        o[2] = b.x | ((uint64_t)b.y << 32); o[3] = b.z | ((uint64_t)b.w << 32);
+    }
+}
+
+__device__ __forceinline__ uint64_t pick4(const uint64_t (&a)[kRows], int i)
+{
+    return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3];
+}
+
+template <int NV>
+__device__ __forceinline__ uint64_t pick_col(const uint64_t (&v)[NV], int c)
+{
+    uint64_t x = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        if (k == c) x = v[k];
+    return x;
+}
+
+// Value-column layouts compiled into the kernel.  GEN: any widths, read with
+// per-element typed loads; otherwise NV columns of W0/W1 bytes read with
+// 16-byte vector loads.
+template <int NV_, int W0_, int W1_, bool GEN_, int C0_ = CONV_RAW, int C1_ = CONV_RAW>
+struct VLayout {
+    static constexpr int NV = NV_;
+    static constexpr bool GEN = GEN_;
+    template <int C>
+    static constexpr int width() { return C == 0 ? W0_ : W1_; }
+    template <int C>
+    static constexpr int conv() { return C == 0 ? C0_ : C1_; }
+};
+
+template <class VL>
+__device__ __forceinline__ void load_tile(const KeyDesc& key, int kb, const AggSpec& spec, int64_t r0,
+                                          uint64_t (&k)[kRows], uint64_t (&v)[VL::NV > 0 ? VL::NV : 1][kRows])
+{
+    if constexpr (VL::GEN) {
+#pragma unroll
+        for (int i = 0; i < kRows; ++i) k[i] = dev::load_raw(key.data, kb, r0 + i);
+#pragma unroll
+        for (int c = 0; c < VL::NV; ++c)
+            if (c < spec.nvcols)
+#pragma unroll
+                for (int i = 0; i < kRows; ++i) v[c][i] = dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, r0 + i);
+    } else {
+        if (kb == 4) vload4<4>(key.data, r0, k);
+        else vload4<8>(key.data, r0, k);
+        if constexpr (VL::NV >= 1) vload4<VL::template width<0>()>(spec.cols[0].data, r0, v[0]);
+        if constexpr (VL::NV >= 2) vload4<VL::template width<1>()>(spec.cols[1].data, r0, v[1]);
+    }
+}
+
+// Layout of the dynamic LDS (8-byte words), T = (Cr + 1) * R slots:
+//   states: nops * T words; op a of slot idx at [a * T + idx]
+//   keys:   T entries of KB bytes (encoded; 0 = empty)
+//   side:   R words, side slot of replica r used
+// Slot (s, r) has index (s << rlog) | r; s = Cr is the side slot holding the
+// key whose LDS encoding is zero.
+template <int KB>
+struct LdsTable {
+    using KT = typename LdsKey<KB>::T;
+    unsigned long long* st;
+    KT* keys;
+    unsigned long long* side;
+    int T, cr, rlog;
+
+    __device__ LdsTable(unsigned long long* smem, int nops, int cr_log2, int rep_log2)
+    {
+        cr = 1 << cr_log2;
+        rlog = rep_log2;
+        T = (cr + 1) << rlog;
+        st = smem;
+        keys = reinterpret_cast<KT*>(st + (size_t)nops * T);
+        side = st + (size_t)nops * T + ((size_t)T * KB + 7) / 8;
+    }
+    __device__ static int words(int nops, int cr_log2, int rep_log2)
+    {
+        const int T = ((1 << cr_log2) + 1) << rep_log2;
+        return nops * T + (T * KB + 7) / 8 + (1 << rep_log2);
+    }
+    // Finds or inserts encoded key e in replica r; -1 when probe_limit slots
+    // are taken by other keys.
+    __device__ __forceinline__ int find(KT e, int r, int probe_limit)
+    {
+        if (e == 0) {
+            if (__hip_atomic_load(&side[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0ull)
+                __hip_atomic_store(&side[r], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return (cr << rlog) | r;
+        }
+        uint32_t h = (KB == 4) ? dev::fmix32((uint32_t)e) : (uint32_t)(dev::fmix64((uint64_t)e) >> 32);
+        h &= (uint32_t)(cr - 1);
+        for (int p = 0; p < probe_limit; ++p) {
+            const int idx = (int)(h << rlog) | r;
+            KT cur = __hip_atomic_load(&keys[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (cur == e) return idx;
+            if (cur == 0) {
+                KT prev = atomicCAS(&keys[idx], (KT)0, e);
+                if (prev == 0 || prev == e) return idx;
+            }
+            h = (h + 1) & (uint32_t)(cr - 1);
+        }
+        return -1;
+    }
+};
+
+template <int KB>
+__device__ __forceinline__ uint64_t lds_to_enc64(typename LdsKey<KB>::T kw, int is_signed)
+{
+    if (KB == 4) return dev::enc64(dev::widen_key((uint32_t)kw ^ dev::kSign32, 4, is_signed));
+    return (uint64_t)kw;
+}
+
+// Canonical values of a row's columns (raw -> 64-bit per ConvCode); the
+// compiled layouts know each column's conversion.
+template <class VL, int NV>
+__device__ __forceinline__ void canon_row(const AggSpec& spec, const uint64_t (&raw)[NV], uint64_t (&cv)[NV])
+{
+    if constexpr (VL::GEN) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) cv[c] = c < spec.nvcols ? dev::canon(raw[c], spec.cols[c].conv) : 0ull;
+    } else {
+        if constexpr (VL::NV >= 1) cv[0] = dev::canon_c<VL::template conv<0>()>(raw[0]);
+        if constexpr (VL::NV >= 2) cv[1] = dev::canon_c<VL::template conv<1>()>(raw[1]);
+        if constexpr (VL::NV == 0) cv[0] = 0ull;
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ uint64_t operand_of(const AggSpec& spec, int a, const uint64_t (&cv)[NV])
+{
+    const OpDesc& od = spec.ops[a];
+    return dev::operand(od, od.col < 0 ? 1ull : pick_col<NV>(cv, od.col));
+}
+
+// One row into the LDS table; returns true when it needs the HBM table.
+template <int KB, int NV>
+__device__ __forceinline__ bool lds_row(LdsTable<KB>& tab, bool use_lds, int r_mask, int lane, const KeyDesc& key,
+                                        const AggSpec& spec, uint64_t kraw, const uint64_t (&cv)[NV],
+                                        bool valid, int probe_limit, int peel_min)
+{
+    using KT = typename LdsKey<KB>::T;
+    int slot = -1;
+    if (use_lds) {
+        // first probe inline: once the table holds the groups it almost always hits
+        KT e = 0;
+        uint32_t h = 0;
+        bool hit = false;
+        if (valid) {
+            e = (KB == 4) ? (KT)((uint32_t)kraw ^ dev::kSign32) : (KT)dev::enc64(dev::widen_key(kraw, KB, key.is_signed));
+            h = ((KB == 4) ? dev::fmix32((uint32_t)e) : (uint32_t)(dev::fmix64((uint64_t)e) >> 32)) & (uint32_t)(tab.cr - 1);
+            const int idx = (int)(h << tab.rlog) | (lane & r_mask);
+            const KT cur = __hip_atomic_load(&tab.keys[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            hit = e != 0 && cur == e;
+            if (hit) slot = idx;
+        }
+        if (__ballot(valid && !hit) != 0ull) {
+            if (valid && !hit) slot = tab.find(e, lane & r_mask, probe_limit);
+        }
+    }
+    bool done = false;
+    if (use_lds && r_mask == 0 && peel_min <= 64) {
+        done = dev::wave_peel(slot >= 0, (uint64_t)(int64_t)slot, spec, peel_min,
+                              [&](int a) { return operand_of<NV>(spec, a, cv); },
+                              [&](uint64_t s) { return &tab.st[(int64_t)s]; }, (int64_t)tab.T);
+    }
+    if (slot >= 0 && !done) dev::apply_ops<NV>(spec, cv, [&](int a) { return &tab.st[(size_t)a * tab.T + slot]; });
+    return valid && slot < 0;
+}
+
+// One row into the HBM table (called by every lane of the wave).
+template <int KB, int NV>
+__device__ __forceinline__ void hbm_row(const GTable& gt, const KeyDesc& key, const AggSpec& spec, bool pending,
+                                        uint64_t kraw, const uint64_t (&cv)[NV], int peel_min)
+{
+    const uint64_t e64 = dev::enc64(dev::widen_key(kraw, KB, key.is_signed));
+    bool done = false;
+    if (peel_min <= 64)
+        done = dev::wave_peel(pending, e64, spec, peel_min, [&](int a) { return operand_of<NV>(spec, a, cv); },
+                              [&](uint64_t t) {
+                                  unsigned long long* g = gt_slot(gt, t);
+                                  return g ? g + 1 : (unsigned long long*)nullptr;
+                              },
+                              (int64_t)1);
+    if (pending && !done) {
+        unsigned long long* g = gt_slot(gt, e64);
+        if (g) dev::apply_ops<NV>(spec, cv, [&](int a) { return g + 1 + a; });
+    }
+}
+
+// Wider value sets need more registers per lane: cap the workgroup at 512.
+template <int NV>
+constexpr int max_block() { return NV <= 2 ? 1024 : 512; }
+
+template <int KB, class VL>
+__global__ __launch_bounds__(max_block<VL::NV>()) void hash_agg_kernel(KeyDesc key, int64_t n, AggSpec spec,
+                                                                        GTable gt, int cr_log2, int rep_log2,
+                                                                        int probe_limit, int peel_min)
+{
+    constexpr int NV = VL::NV > 0 ? VL::NV : 1;
+    using KT = typename LdsKey<KB>::T;
+    extern __shared__ unsigned long long smem[];
+    const bool use_lds = cr_log2 >= 0;
+    const int nops = spec.nops;
+    const int lane = dev::lane_id();
+    LdsTable<KB> tab(smem, nops, use_lds ? cr_log2 : 0, use_lds ? rep_log2 : 0);
+    if (use_lds) {
+        const int words = LdsTable<KB>::words(nops, cr_log2, rep_log2);
+        for (int i = threadIdx.x; i < words; i += blockDim.x) smem[i] = 0ull;
+        __syncthreads();
+    }
+    const int r_mask = (1 << (use_lds ? rep_log2 : 0)) - 1;
+
+    // ---- full tiles: vector loads, next tile in flight while this one runs ----
+    const int64_t tile = (int64_t)blockDim.x * kRows;
+    const int64_t full = n / tile;
+    int64_t t = blockIdx.x;
+    uint64_t kcur[kRows], vcur[NV][kRows];
+    if (t < full) load_tile<VL>(key, KB, spec, t * tile + (int64_t)threadIdx.x * kRows, kcur, vcur);
+    for (; t < full; t += gridDim.x) {
+        uint64_t knxt[kRows], vnxt[NV][kRows];
+        const int64_t tn = t + gridDim.x;
+        if (tn < full) load_tile<VL>(key, KB, spec, tn * tile + (int64_t)threadIdx.x * kRows, knxt, vnxt);
+        uint32_t spill = 0;
+#pragma unroll
+        for (int i = 0; i < kRows; ++i) {
+            uint64_t raw[NV], cv[NV];
+#pragma unroll
+            for (int c = 0; c < NV; ++c) raw[c] = vcur[c][i];
+            canon_row<VL, NV>(spec, raw, cv);
+            if (lds_row<KB, NV>(tab, use_lds, r_mask, lane, key, spec, kcur[i], cv, true, probe_limit, peel_min))
+                spill |= 1u << i;
+        }
+        if (__ballot(spill != 0) != 0ull) {
+#pragma unroll 1
+            for (int i = 0; i < kRows; ++i) {
+                const bool pend = (spill >> i) & 1u;
+                if (__ballot(pend) == 0ull) continue;
+                uint64_t raw[NV], cv[NV];
+#pragma unroll
+                for (int c = 0; c < NV; ++c) raw[c] = pick4(vcur[c], i);
+                canon_row<VL, NV>(spec, raw, cv);
+                hbm_row<KB, NV>(gt, key, spec, pend, pick4(kcur, i), cv, peel_min);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kRows; ++i) {
+            kcur[i] = knxt[i];
+#pragma unroll
+            for (int c = 0; c < NV; ++c) vcur[c][i] = vnxt[c][i];
+        }
+    }
+    // ---- tail rows: one row per lane, typed loads ----
+    for (int64_t b = full * tile + (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = b + threadIdx.x;
+        const bool valid = r < n;
+        const uint64_t kraw = valid ? dev::load_raw(key.data, KB, r) : 0ull;
+        uint64_t raw[NV], cv[NV];
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+            raw[c] = (valid && c < spec.nvcols) ? dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, r) : 0ull;
+        canon_row<VL, NV>(spec, raw, cv);
+        const bool sp = lds_row<KB, NV>(tab, use_lds, r_mask, lane, key, spec, kraw, cv, valid, probe_limit, peel_min);
+        if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kraw, cv, peel_min);
+    }
+
+    if (!use_lds) return;
+    __syncthreads();
+    // merge replicas 1..R-1 into replica 0 (LDS atomics)
+    const int R = 1 << rep_log2;
+    if (R > 1) {
+        for (int idx = threadIdx.x; idx < tab.T; idx += blockDim.x) {
+            const int r = idx & r_mask;
+            if (r == 0) continue;
+            const int s = idx >> rep_log2;
+            int dst;
+            KT kw = 0;
+            if (s == tab.cr) {
+                if (tab.side[r] == 0ull) continue;
+                dst = tab.find((KT)0, 0, 1);
+            } else {
+                kw = tab.keys[idx];
+                if (kw == 0) continue;
+                dst = tab.find(kw, 0, tab.cr);
+            }
+            auto src = [&](int a) { return tab.st[(size_t)a * tab.T + idx]; };
+            if (dst >= 0) {
+                dev::merge_states(spec, [&](int a) { return &tab.st[(size_t)a * tab.T + dst]; }, src);
+            } else {
+                // replica 0 is full: merge this slot straight into HBM
+                unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed));
+                if (g) dev::merge_states(spec, [&](int a) { return g + 1 + a; }, src);
+            }
+        }
+        __syncthreads();
+    }
+    // flush replica 0 into the HBM table
+    for (int s = threadIdx.x; s <= tab.cr; s += blockDim.x) {
+        const int idx = s << rep_log2;
+        KT kw;
+        if (s == tab.cr) {
+            if (tab.side[0] == 0ull) continue;
+            kw = 0;
+        } else {
+            kw = tab.keys[idx];
+            if (kw == 0) continue;
+        }
+        unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed));
+        if (!g) continue;
+        dev::merge_states(spec, [&](int a) { return g + 1 + a; }, [&](int a) { return tab.st[(size_t)a * tab.T + idx]; });
+    }
+}
+
+__global__ void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter)
+{
+    const uint64_t total = gt.mask + 2;  // slots + side slot
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
+        uint64_t s = base + threadIdx.x;
+        bool occ = false;
+        const unsigned long long* slot = nullptr;
+        uint64_t enc = 0;
+        if (s <= gt.mask) {
+            slot = gt.base + s * (uint64_t)gt.W;
+            enc = slot[0];
+            occ = enc != 0;
+        } else if (s == gt.mask + 1) {
+            slot = gt.side;
+            enc = 0;
+            occ = gt.meta[1] != 0;
+        }
+        int64_t idx = dev::wave_claim(counter, occ);
+        if (occ) dev::emit_group(os, idx, enc, slot + 1, 1);
+    }
+}
+
+template <int KB, class VL>
+void launch_agg(const DeviceInfo& di, const HashTuning& t, const KeyDesc& key, int64_t n, const AggSpec& spec,
+                const GTable& gt, const HashLayout& L, hipStream_t s)
+{
+    auto kern = hash_agg_kernel<KB, VL>;
+    static bool attr_set = false;  // per instantiation
+    if (!attr_set) {
+        EG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, di.max_lds));
+        attr_set = true;
+    }
+    const int block = std::min(t.block, max_block<VL::NV>());
+    int per_cu = t.blocks_per_cu;
+    if (per_cu <= 0) {
+        EG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, L.lds_bytes));
+        per_cu = std::max(per_cu, 1);
+    }
+    const int64_t tile = (int64_t)block * kRows;
+    const int64_t want = (n + tile - 1) / tile;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)di.num_cus * per_cu));
+    KernelTimer timer("hash_agg", s);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), L.lds_bytes, s, key, n, spec, gt, L.cr_log2,
+                       L.rep_log2, L.probe_limit, L.peel_min);
+    EG_LAUNCH_CHECK();
+}
+
+constexpr int kind_width(int k) { return k == 0 ? 8 : 4; }
+constexpr int kind_conv(int k) { return k == 1 ? CONV_SEXT32 : k == 3 ? CONV_F32 : CONV_RAW; }
+
+template <class F>
+void with_kind(int k, F&& f)
+{
+    switch (k) {
+    case 0: f(std::integral_constant<int, 0>{}); break;
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    default: f(std::integral_constant<int, 3>{}); break;
+    }
+}
+
+// Picks the compiled value layout: vector loads for up to two 4/8-byte
+// columns on 16-byte aligned data, the generic typed-load kernel otherwise.
+template <int KB>
+void dispatch_layout(bool vec, const DeviceInfo& di, const HashTuning& t, const KeyDesc& key, int64_t n,
+                     const AggSpec& spec, const GTable& gt, const HashLayout& L, hipStream_t s)
+{
+    const int nv = spec.nvcols;
+    auto w = [&](int c) { return spec.cols[c].bytes; };
+    bool fast = vec && nv <= 2;
+    for (int c = 0; c < nv; ++c) fast = fast && (w(c) == 4 || w(c) == 8);
+    if (fast) {
+        // column kind: 0 = 8-byte, 1 = int32, 2 = uint32, 3 = float32
+        auto kind = [&](int c) {
+            const ColDesc& d = spec.cols[c];
+            return d.bytes == 8 ? 0 : d.conv == CONV_SEXT32 ? 1 : d.conv == CONV_F32 ? 3 : 2;
+        };
+        if (nv == 0) return launch_agg<KB, VLayout<0, 4, 4, false>>(di, t, key, n, spec, gt, L, s);
+        if (nv == 1)
+            return with_kind(kind(0), [&](auto k0) {
+                constexpr int K0 = decltype(k0)::value;
+                launch_agg<KB, VLayout<1, kind_width(K0), 4, false, kind_conv(K0)>>(di, t, key, n, spec, gt, L, s);
+            });
+        return with_kind(kind(0), [&](auto k0) {
+            with_kind(kind(1), [&](auto k1) {
+                constexpr int K0 = decltype(k0)::value, K1 = decltype(k1)::value;
+                launch_agg<KB, VLayout<2, kind_width(K0), kind_width(K1), false, kind_conv(K0), kind_conv(K1)>>(
+                    di, t, key, n, spec, gt, L, s);
+            });
+        });
+    }
+    if (nv <= 2) return launch_agg<KB, VLayout<2, 0, 0, true>>(di, t, key, n, spec, gt, L, s);
+    if (nv <= 4) return launch_agg<KB, VLayout<4, 0, 0, true>>(di, t, key, n, spec, gt, L, s);
+    return launch_agg<KB, VLayout<8, 0, 0, true>>(di, t, key, n, spec, gt, L, s);
+}
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace
+
+HashLayout plan_hash_layout(int nops, int key_bytes, int64_t group_bound, const HashTuning& t)
+{
+    const DeviceInfo& di = device_info();
+    HashLayout L;
+    auto bytes_of = [&](int cr_log2, int rep_log2) {
+        const size_t T = (size_t)((1 << cr_log2) + 1) << rep_log2;
+        return (size_t)nops * T * 8 + (T * key_bytes + 7) / 8 * 8 + ((size_t)1 << rep_log2) * 8;
+    };
+    const size_t budget = (size_t)di.max_lds;
+    // replicas: the most (<= 64) that still give every replica >= 2x the group bound
+    int best_c = -1, best_r = 0;
+    for (int rl = 6; rl >= 0; --rl) {
+        if (t.replicas > 0 && (1 << rl) > t.replicas) continue;
+        int cl = -1;
+        for (int c = 14; c >= 4; --c)
+            if (bytes_of(c, rl) <= budget && (t.lds_slots == 0 || ((1 << c) << rl) <= t.lds_slots)) {
+                cl = c;
+                break;
+            }
+        if (cl < 0) continue;
+        if (rl == 0 || (int64_t)(1 << cl) >= 2 * group_bound) {
+            best_c = cl;
+            best_r = rl;
+            break;
+        }
+    }
+    if (best_c >= 0 && group_bound <= (int64_t)t.lds_max_ratio * ((int64_t)1 << best_c)) {
+        L.cr_log2 = best_c;
+        L.rep_log2 = best_r;
+        L.lds_bytes = bytes_of(best_c, best_r);
+    }
+    // A table much smaller than the group count still catches the hot keys
+    // (they arrive early); keep the probing short so cold rows spill cheaply.
+    const int64_t slots = (int64_t)1 << std::max(best_c, 0);
+    L.probe_limit = group_bound > 4 * slots ? 2 : group_bound > slots ? 4 : 16;
+    L.peel_min = t.peel_min;
+    return L;
+}
+
+Result run_hash(const Plan& plan, int64_t group_bound, hipStream_t s, const HashTuning& t)
+{
+    const DeviceInfo& di = device_info();
+    const int64_t n = plan.rows;
+    const int nops = plan.spec.nops;
+    const int W = table_words_per_slot(nops);
+    const int KB = plan.key.bytes;
+    const HashLayout L = plan_hash_layout(nops, KB, group_bound, t);
+
+    bool vec = aligned16(plan.key.data);
+    for (int c = 0; c < plan.spec.nvcols; ++c) vec = vec && aligned16(plan.spec.cols[c].data);
+
+    int64_t bound = std::max<int64_t>(1, std::min<int64_t>(group_bound, n));
+    int64_t slots = t.table_slots > 0 ? t.table_slots : 1024;
+    while (slots < 2 * bound) slots <<= 1;
+
+    for (;;) {
+        HbmTable ht(slots, W, s);
+        if (KB == 4) dispatch_layout<4>(vec, di, t, plan.key, n, plan.spec, ht.gt, L, s);
+        else dispatch_layout<8>(vec, di, t, plan.key, n, plan.spec, ht.gt, L, s);
+        int64_t meta[3];
+        ht.read_meta(meta, s);
+        if (meta[2] != 0) {
+            // The table filled up: the group bound was too small.  Retry with a
+            // table sized for every row being its own group.
+            if (slots >= 2 * n) raise(EG_ERR_CAPACITY, "hash table overflow");
+            int64_t ns = slots;
+            while (ns < 2 * n) ns <<= 1;
+            slots = ns;
+            continue;
+        }
+        const int64_t groups = meta[0] + (meta[1] != 0 ? 1 : 0);
+        Result res;
+        OutSpec os;
+        alloc_outputs(plan, groups, s, res, os);
+        if (groups > 0) {
+            DevBuf counter(8, s);
+            fill_zero(counter.get(), 8, s);
+            ht.compact(os, counter.get<unsigned long long>(), s);
+        }
+        EG_HIP(hipStreamSynchronize(s));
+        return res;
+    }
+}
+
+// ---- HbmTable ---------------------------------------------------------------------
+int table_words_per_slot(int nops)
+{
+    int p = 1;
+    while (p < 1 + nops) p <<= 1;
+    return p;
+}
+
+HbmTable::HbmTable(int64_t slots, int W, hipStream_t s)
+    : buf((size_t)(slots * W + W + 4) * 8, s)
+{
+    const size_t words = (size_t)(slots * W + W + 4);
+    fill_zero(buf.get(), words * 8, s);
+    gt.base = buf.get<unsigned long long>();
+    gt.side = gt.base + (size_t)slots * W;
+    gt.meta = gt.side + W;
+    gt.mask = (uint64_t)slots - 1;
+    gt.W = W;
+}
+
+void HbmTable::read_meta(int64_t* out3, hipStream_t s)
+{
+    int64_t* h = pinned_i64(4);
+    EG_HIP(hipMemcpyAsync(h, gt.meta, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    out3[0] = h[0];
+    out3[1] = h[1];
+    out3[2] = h[2];
+}
+
+void HbmTable::compact(const OutSpec& os, unsigned long long* counter, hipStream_t s)
+{
+    const DeviceInfo& di = device_info();
+    const uint64_t total = gt.mask + 2;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + 255) / 256), (int64_t)di.num_cus * 8));
+    KernelTimer timer("hash_compact", s);
+    hipLaunchKernelGGL(hash_compact_kernel, dim3((unsigned)grid), dim3(256), 0, s, gt, os, counter);
+    EG_LAUNCH_CHECK();
+}
+
+}  // namespace eg

@@ -1,0 +1,349 @@
+// runtime.hip -- errors, type helpers, the device memory manager, kernel
+// timing and fill kernels of libeiger_groupby.so.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+#include "eg_internal.hpp"
+
+namespace eg {
+
+// ---- errors ---------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+const char* last_error_cstr() { return g_last_error.c_str(); }
+
+// ---- types ---------------------------------------------------------------------
+int type_size(int32_t t)
+{
+    switch (t) {
+    case EG_INT8: case EG_UINT8: case EG_BOOL8: return 1;
+    case EG_INT16: case EG_UINT16: return 2;
+    case EG_INT32: case EG_UINT32: case EG_FLOAT32: return 4;
+    case EG_INT64: case EG_UINT64: case EG_FLOAT64: return 8;
+    default: return 0;
+    }
+}
+
+bool is_signed_int(int32_t t) { return t == EG_INT8 || t == EG_INT16 || t == EG_INT32 || t == EG_INT64; }
+bool is_unsigned_int(int32_t t) { return t == EG_UINT8 || t == EG_UINT16 || t == EG_UINT32 || t == EG_UINT64; }
+
+// agg_output_data_type (operators/groupby/aggregate.cu:20-53) with the SUM
+// widening of agg_func.cuh:18-36.
+int32_t agg_output_type(int32_t agg_kind, int32_t type)
+{
+    if (agg_kind == EG_AGG_COUNT_ALL) return EG_INT64;
+    bool numeric = is_int(type) || is_float(type);
+    if (!numeric) return EG_TYPE_EMPTY;
+    switch (agg_kind) {
+    case EG_AGG_AVG: return EG_FLOAT64;
+    case EG_AGG_SUM:
+        if (is_signed_int(type)) return EG_INT64;
+        if (is_unsigned_int(type)) return EG_UINT64;
+        return type;
+    case EG_AGG_MAX: case EG_AGG_MIN: case EG_AGG_ANY: return type;
+    default: return EG_TYPE_EMPTY;
+    }
+}
+
+// ---- fill kernels ----------------------------------------------------------------
+__global__ void fill_zero_kernel(uint4* p, int64_t n16, uint8_t* tail, int64_t ntail)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) p[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (blockIdx.x == 0 && (int64_t)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+void fill_zero(void* p, size_t bytes, hipStream_t s)
+{
+    if (!bytes) return;
+    int64_t n16 = (int64_t)(bytes / 16);
+    int64_t ntail = (int64_t)(bytes - (size_t)n16 * 16);
+    int64_t blocks = std::min<int64_t>(std::max<int64_t>((n16 + 255) / 256, 1), 8192);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)p, n16,
+                       (uint8_t*)p + n16 * 16, ntail);
+    EG_LAUNCH_CHECK();
+}
+
+// ---- device facts ----------------------------------------------------------------
+const DeviceInfo& device_info()
+{
+    static std::mutex mu;
+    static std::map<int, DeviceInfo> cache;
+    int dev = 0;
+    EG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    DeviceInfo d{};
+    d.device = dev;
+    EG_HIP(hipDeviceGetAttribute(&d.num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    EG_HIP(hipDeviceGetAttribute(&d.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    EG_HIP(hipDeviceGetAttribute(&d.warp_size, hipDeviceAttributeWarpSize, dev));
+    if (d.warp_size != 64) raise(EG_ERR_UNSUPPORTED, "wave64 device required (gfx950)");
+    return cache.emplace(dev, d).first->second;
+}
+
+// ---- memory manager (mirrors gamax::MemoryManager, memory/mem_manager.hpp:12-158) --
+namespace {
+struct Alloc {
+    size_t bytes;
+    hipStream_t stream;
+};
+
+struct Pool {
+    char* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_list;  // offset -> length, coalesced
+};
+
+std::mutex g_mm_mu;
+int g_mode = EG_MM_ASYNC;
+std::unordered_map<void*, Alloc> g_allocs;
+int64_t g_cur = 0, g_peak = 0;
+Pool g_pool;
+bool g_pool_threshold_set = false;
+
+constexpr size_t kAlign = 256;
+
+void* pool_alloc(size_t bytes)
+{
+    bytes = (bytes + kAlign - 1) / kAlign * kAlign;
+    for (auto it = g_pool.free_list.begin(); it != g_pool.free_list.end(); ++it) {
+        if (it->second >= bytes) {
+            size_t off = it->first, len = it->second;
+            g_pool.free_list.erase(it);
+            if (len > bytes) g_pool.free_list.emplace(off + bytes, len - bytes);
+            return g_pool.base + off;
+        }
+    }
+    return nullptr;
+}
+
+void pool_free(void* p, size_t bytes)
+{
+    bytes = (bytes + kAlign - 1) / kAlign * kAlign;
+    size_t off = (size_t)((char*)p - g_pool.base);
+    auto it = g_pool.free_list.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != g_pool.free_list.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        g_pool.free_list.erase(nx);
+    }
+    if (it != g_pool.free_list.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            g_pool.free_list.erase(it);
+        }
+    }
+}
+}  // namespace
+
+void* mm_alloc(size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) bytes = kAlign;
+    std::lock_guard<std::mutex> lk(g_mm_mu);
+    void* p = nullptr;
+    if (g_mode == EG_MM_POOL) {
+        p = pool_alloc(bytes);
+        if (!p) throw std::bad_alloc();
+    } else {
+        if (!g_pool_threshold_set) {
+            int dev = 0;
+            hipMemPool_t mp;
+            if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&mp, dev) == hipSuccess) {
+                uint64_t thr = UINT64_MAX;
+                (void)hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &thr);
+            }
+            g_pool_threshold_set = true;
+        }
+        hipError_t e = hipMallocAsync(&p, bytes, s);
+        if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+            (void)hipGetLastError();
+            throw std::bad_alloc();
+        }
+        if (e != hipSuccess) raise(EG_ERR_DEVICE, std::string("hipMallocAsync: ") + hipGetErrorString(e));
+    }
+    g_allocs[p] = Alloc{bytes, s};
+    g_cur += (int64_t)bytes;
+    g_peak = std::max(g_peak, g_cur);
+    return p;
+}
+
+void mm_free(void* p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_mm_mu);
+    auto it = g_allocs.find(p);
+    if (it == g_allocs.end()) raise(EG_ERR_INVALID, "eg_free: pointer not allocated by this library");
+    Alloc a = it->second;
+    g_allocs.erase(it);
+    g_cur -= (int64_t)a.bytes;
+    if (g_mode == EG_MM_POOL) pool_free(p, a.bytes);
+    else EG_HIP(hipFreeAsync(p, a.stream));
+}
+
+void mm_usage(int64_t* cur, int64_t* peak)
+{
+    std::lock_guard<std::mutex> lk(g_mm_mu);
+    if (cur) *cur = g_cur;
+    if (peak) *peak = g_peak;
+}
+
+void mm_set_mode(int mode, int64_t pool_bytes)
+{
+    std::lock_guard<std::mutex> lk(g_mm_mu);
+    if (!g_allocs.empty()) raise(EG_ERR_INVALID, "memory mode can only change while nothing is allocated");
+    if (mode != EG_MM_ASYNC && mode != EG_MM_POOL) raise(EG_ERR_INVALID, "unknown memory mode");
+    if (g_pool.base) {
+        EG_HIP(hipFree(g_pool.base));
+        g_pool = Pool{};
+    }
+    if (mode == EG_MM_POOL) {
+        if (pool_bytes <= 0) raise(EG_ERR_INVALID, "pool_bytes must be positive");
+        size_t sz = ((size_t)pool_bytes + kAlign - 1) / kAlign * kAlign;
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, sz);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw std::bad_alloc();
+        }
+        g_pool.base = (char*)p;
+        g_pool.size = sz;
+        g_pool.free_list.emplace(0, sz);
+    }
+    g_mode = mode;
+    g_peak = g_cur;
+}
+
+int64_t* pinned_i64(int count)
+{
+    static thread_local int64_t* buf = nullptr;
+    static thread_local int cap = 0;
+    if (count > cap) {
+        if (buf) (void)hipHostFree(buf);
+        buf = nullptr;
+        int c = std::max(count, 64);
+        EG_HIP(hipHostMalloc((void**)&buf, sizeof(int64_t) * c, hipHostMallocDefault));
+        cap = c;
+    }
+    return buf;
+}
+
+// ---- kernel timing -----------------------------------------------------------------
+namespace {
+struct TimingRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+std::mutex g_t_mu;
+bool g_timing = false;
+std::vector<TimingRec> g_recs;
+}  // namespace
+
+bool timing_enabled() { return g_timing; }
+
+KernelTimer::KernelTimer(const char* name, hipStream_t s) : name_(name), s_(s)
+{
+    if (!g_timing) return;
+    EG_HIP(hipEventCreate(&a_));
+    EG_HIP(hipEventCreate(&b_));
+    EG_HIP(hipEventRecord(a_, s_));
+}
+
+KernelTimer::~KernelTimer()
+{
+    if (!a_) return;
+    if (hipEventRecord(b_, s_) != hipSuccess) return;
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    g_recs.push_back(TimingRec{name_, a_, b_});
+}
+
+int timing_enable(int on)
+{
+    g_timing = on != 0;
+    return EG_OK;
+}
+
+int timing_reset()
+{
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    for (auto& r : g_recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_recs.clear();
+    return EG_OK;
+}
+
+void timing_query(const char* name, int64_t* launches, double* total_ms)
+{
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    int64_t n = 0;
+    double tot = 0.0;
+    for (auto& r : g_recs) {
+        if (r.name != name) continue;
+        EG_HIP(hipEventSynchronize(r.b));
+        float ms = 0.f;
+        EG_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        tot += ms;
+        ++n;
+    }
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = tot;
+}
+
+}  // namespace eg
+
+// ---- bandwidth probe ------------------------------------------------------------------
+namespace eg {
+namespace {
+__global__ __launch_bounds__(256) void probe_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                         int64_t n16)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+}  // namespace
+}  // namespace eg
+
+extern "C" int eg_probe_copy(int64_t bytes, int32_t iters, void* stream, double* ms)
+{
+    return eg::guard([&] {
+        if (bytes <= 0 || iters <= 0 || !ms) eg::raise(EG_ERR_INVALID, "bad arguments");
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        const int64_t n16 = bytes / 16;
+        eg::DevBuf a((size_t)n16 * 16, s), b((size_t)n16 * 16, s);
+        eg::fill_zero(a.get(), (size_t)n16 * 16, s);
+        const eg::DeviceInfo& di = eg::device_info();
+        const int64_t grid = std::min<int64_t>((n16 + 255) / 256, (int64_t)di.num_cus * 16);
+        hipEvent_t e0, e1;
+        EG_HIP(hipEventCreate(&e0));
+        EG_HIP(hipEventCreate(&e1));
+        hipLaunchKernelGGL(eg::probe_copy_kernel, dim3((unsigned)grid), dim3(256), 0, s, a.get<const uint4>(),
+                           b.get<uint4>(), n16);
+        EG_LAUNCH_CHECK();
+        EG_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < iters; ++i) {
+            eg::KernelTimer t("probe_copy", s);
+            hipLaunchKernelGGL(eg::probe_copy_kernel, dim3((unsigned)grid), dim3(256), 0, s, a.get<const uint4>(),
+                               b.get<uint4>(), n16);
+            EG_LAUNCH_CHECK();
+        }
+        EG_HIP(hipEventRecord(e1, s));
+        EG_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        EG_HIP(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = (double)t / iters;
+    });
+}

@@ -1,0 +1,127 @@
+// stats.hip -- distinct-count estimate and the rule-based strategy choice.
+//
+// compute_ndv_approx (stats/stats_compute.cu:103-115,186-196) is a cuco
+// HyperLogLog; this is a HyperLogLog of our own (Flajolet et al. 2007, with
+// the linear-counting small-range correction), 2^14 registers, one read of the
+// column.  Each workgroup keeps its registers in LDS (32-bit, so LDS atomicMax
+// applies), writes them out, and a second kernel takes the per-register max.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "eg_device.hpp"
+#include "eg_internal.hpp"
+
+namespace eg {
+namespace {
+
+constexpr int kP = 14;
+constexpr int kM = 1 << kP;
+
+__global__ __launch_bounds__(256) void hll_kernel(const void* data, int bytes, int64_t n,
+                                                  uint8_t* __restrict__ partial)
+{
+    __shared__ uint32_t reg[kM];
+    for (int i = threadIdx.x; i < kM; i += blockDim.x) reg[i] = 0u;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint64_t raw = dev::load_raw(data, bytes, i);
+        uint64_t h = dev::fmix64(raw ^ 0x9e3779b97f4a7c15ull);
+        uint32_t idx = (uint32_t)(h >> (64 - kP));
+        uint64_t w = (h << kP) | (1ull << (kP - 1));
+        uint32_t rho = (uint32_t)__clzll((long long)w) + 1u;
+        atomicMax(&reg[idx], rho);
+    }
+    __syncthreads();
+    uint8_t* out = partial + (size_t)blockIdx.x * kM;
+    for (int i = threadIdx.x; i < kM; i += blockDim.x) out[i] = (uint8_t)reg[i];
+}
+
+__global__ void hll_merge_kernel(const uint8_t* __restrict__ partial, int parts, uint8_t* __restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kM) return;
+    uint8_t m = 0;
+    for (int p = 0; p < parts; ++p) m = std::max(m, partial[(size_t)p * kM + i]);
+    out[i] = m;
+}
+
+}  // namespace
+
+int64_t ndv_approx(const void* data, int32_t type, int64_t n, hipStream_t s)
+{
+    const int bytes = type_size(type);
+    if (!bytes) raise(EG_ERR_UNSUPPORTED, "compute_ndv_approx: unsupported column type");
+    if (n <= 0) return 0;
+    const DeviceInfo& di = device_info();
+    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>((n + 4095) / 4096, (int64_t)di.num_cus * 2));
+    DevBuf partial((size_t)parts * kM + kM, s);
+    uint8_t* pp = partial.get<uint8_t>();
+    {
+        KernelTimer timer("hll", s);
+        hipLaunchKernelGGL(hll_kernel, dim3(parts), dim3(256), 0, s, data, bytes, n, pp);
+        EG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(hll_merge_kernel, dim3(kM / 256), dim3(256), 0, s, pp, parts, pp + (size_t)parts * kM);
+        EG_LAUNCH_CHECK();
+    }
+    std::vector<uint8_t> reg(kM);
+    EG_HIP(hipMemcpyAsync(reg.data(), pp + (size_t)parts * kM, kM, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    double sum = 0.0;
+    int zeros = 0;
+    for (int i = 0; i < kM; ++i) {
+        sum += std::ldexp(1.0, -(int)reg[i]);
+        zeros += reg[i] == 0;
+    }
+    const double alpha = 0.7213 / (1.0 + 1.079 / kM);
+    double e = alpha * (double)kM * (double)kM / sum;
+    if (e <= 2.5 * kM && zeros > 0) e = (double)kM * std::log((double)kM / (double)zeros);
+    int64_t r = (int64_t)std::llround(e);
+    return std::max<int64_t>(1, std::min<int64_t>(r, n));
+}
+
+}  // namespace eg
+
+extern "C" {
+
+int eg_compute_ndv_approx(eg_column_view col, void* stream, int64_t* out_ndv)
+{
+    return eg::guard([&] {
+        if (!out_ndv) eg::raise(EG_ERR_INVALID, "out_ndv is null");
+        const int bytes = eg::type_size(col.type);
+        if (!bytes) eg::raise(EG_ERR_UNSUPPORTED, "compute_ndv_approx: unsupported column type");
+        if (col.size > 0 && !col.data) eg::raise(EG_ERR_INVALID, "null column data");
+        const void* d = col.data ? (const char*)col.data + col.offset * bytes : nullptr;
+        *out_ndv = eg::ndv_approx(d, col.type, col.size, reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+// auto_select_groupby_algo (operators/groupby/groupby.cu:68-119).
+int eg_auto_select_algo(const eg_column_view* keys, int64_t num_keys, eg_groupby_hint* hint, void* stream)
+{
+    return eg::guard([&] {
+        if (!hint) eg::raise(EG_ERR_INVALID, "hint is null");
+        if (hint->groupby_algo != EG_ALGO_AUTO) return;
+        if (num_keys <= 0 || !keys) eg::raise(EG_ERR_INVALID, "no key columns");
+        for (int64_t i = 0; i < num_keys; ++i)
+            if (keys[i].type == EG_STRING) {
+                hint->groupby_algo = EG_ALGO_HASH;
+                return;
+            }
+        if (num_keys != 1) {
+            hint->groupby_algo = EG_ALGO_HASH;
+            return;
+        }
+        int64_t est = 0;
+        int rc = eg_compute_ndv_approx(keys[0], stream, &est);
+        if (rc != EG_OK) eg::raise(rc, eg_last_error());
+        const bool four = eg::type_size(keys[0].type) == 4;
+        const int64_t thr = four ? ((int64_t)1 << 14) : ((int64_t)1 << 18);
+        hint->groupby_algo = est <= thr ? EG_ALGO_HASH : EG_ALGO_PARTITION;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,322 @@
+"""GPU parity tests: the HIP group-by through the C ABI against the oracle on
+the same seeded inputs (bit-exact for integer keys, COUNT and integer
+SUM/MIN/MAX; float32 SUM within 1e-6 relative of the fp64 oracle), the
+reference's own fixtures, and size-independent properties at full size."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from eiger_amd import gamax as G
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = json.load(open(os.path.join(HERE, "golden", "reference_cases.json")))
+
+A = G.aggregation_id
+STRATEGIES = {
+    "auto": G.groupby,
+    "hash": G.hash_groupby,
+}
+
+
+def run(fn, keys, vals, reqs, hint=None):
+    kc = G.make_column_from_host(keys)
+    vcs = [G.make_column_from_host(v) for v in vals]
+    t = fn([kc], vcs, reqs, hint or G.groupby_hint())
+    out = t.to_numpy()
+    t.free()
+    return out
+
+
+def sorted_by_key(cols):
+    order = np.argsort(cols[0], kind="stable")
+    return [c[order] for c in cols]
+
+
+def assert_same(got, exp, float_rtol=1e-6):
+    assert len(got) == len(exp)
+    for j, (g, e) in enumerate(zip(got, exp)):
+        assert g.dtype == e.dtype, (j, g.dtype, e.dtype)
+        assert g.shape == e.shape, (j, g.shape, e.shape)
+        if g.dtype.kind == "f":
+            np.testing.assert_allclose(g, e, rtol=float_rtol, atol=0, err_msg=f"column {j}")
+        else:
+            np.testing.assert_array_equal(g, e, err_msg=f"column {j}")
+
+
+# ------------------------------------------------------------------ generators
+@pytest.mark.parametrize("t", [np.int32, np.int64, np.uint32])
+def test_device_generators_match_oracle(t):
+    n = 100_003
+    tid = G.to_type_id(t)
+    k = G.gen_keys_uniform(n, 777, seed=9, t=tid).to_numpy()
+    np.testing.assert_array_equal(k, O.gen_keys_uniform(n, 777, seed=9, dtype=t))
+    k2 = G.gen_keys_uniform(n, 1 << 20, seed=9, t=tid, begin=500, count=1000).to_numpy()
+    np.testing.assert_array_equal(k2, O.gen_keys_uniform(n, 1 << 20, seed=9, dtype=t, begin=500, count=1000))
+    z = G.gen_keys_zipf(n, 5000, 1.2, seed=3, t=tid).to_numpy()
+    np.testing.assert_array_equal(z, O.gen_keys_zipf(n, O.zipf_cdf(5000, 1.2), seed=3, dtype=t))
+    for dist, vt, sid in ((G.val_dist.UNIFORM_I64, np.int64, 2), (G.val_dist.UNIFORM_F32, np.float32, 3),
+                          (G.val_dist.UNIFORM_I32_1000, np.int32, 4), (G.val_dist.SEQ, np.int64, 5)):
+        v = G.gen_values(n, dist, G.to_type_id(vt), seed=11, stream_id=sid).to_numpy()
+        np.testing.assert_array_equal(v, O.gen_values(n, int(dist), vt, seed=11, stream_id=sid))
+
+
+# ------------------------------------------------------------------ config 1
+@pytest.mark.parametrize("name,fn", list(STRATEGIES.items()))
+def test_plain_hash_small_config_bit_exact(name, fn):
+    """2^20 rows, int32 keys in [0,1024), int32 values in [0,1000), COUNT + SUM."""
+    n = 1 << 20
+    keys = O.gen_keys_uniform(n, 1024, seed=42)
+    vals = O.gen_values(n, O.VAL_UNIFORM_I32_1000, np.int32, seed=42, stream_id=4)
+    reqs = [(-1, A.COUNT_ALL), (0, A.SUM)]
+    got = sorted_by_key(run(fn, keys, [vals], reqs, G.groupby_hint(groups_upper_bound=1024)))
+    exp = O.groupby(keys, [vals], reqs)
+    assert_same(got, exp)
+
+
+# ------------------------------------------------------------------ reference fixtures
+@pytest.mark.parametrize("name,fn", list(STRATEGIES.items()))
+@pytest.mark.parametrize("case", FIX["groupby"], ids=lambda c: c["name"])
+def test_reference_fixture(case, name, fn):
+    keys = np.array(case["keys"], dtype=case["key_type"])
+    vals = [np.array(v, dtype=t) for v, t in zip(case["vals"], case["val_types"])]
+    reqs = [tuple(r) for r in case["requests"]]
+    if name != "auto" and any(k == A.AVG for _, k in reqs):
+        pytest.skip("AVG is rewritten only by the top-level groupby")
+    hint = G.groupby_hint(groups_upper_bound=case["groups_upper_bound"], agg_count=case["agg_count"])
+    if name != "auto" and case["agg_count"]:
+        pytest.skip("agg_count is applied by the top-level groupby")
+    got = sorted_by_key(run(fn, keys, vals, reqs, hint))
+    assert got[0].tolist() == case["expected_keys"]
+    for g, e, et in zip(got[1:], case["expected_cols"], case["expected_types"]):
+        assert g.dtype == np.dtype(et)
+        if case["tol"] is None:
+            assert g.tolist() == e
+        else:
+            np.testing.assert_allclose(g.astype(np.float64), np.array(e, np.float64), rtol=0,
+                                       atol=max(case["tol"], 1e-6))
+
+
+# ------------------------------------------------------------------ all kinds and types
+def _rand_case(seed, kt, n, ngroups):
+    rng = np.random.default_rng(seed)
+    info = np.iinfo(kt)
+    lo = info.min if info.min < 0 else 0
+    pool = np.unique(rng.integers(lo // 2, info.max // 2, ngroups, dtype=np.int64).astype(kt))
+    pool[: min(4, len(pool))] = np.array([info.min, info.max, 0, info.min + 1], dtype=kt)[: min(4, len(pool))]
+    keys = pool[rng.integers(0, len(pool), n)]
+    vals = [rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64),
+            rng.integers(-(1 << 31), (1 << 31) - 1, n).astype(np.int32),
+            rng.integers(0, 1 << 32, n).astype(np.uint32),
+            rng.integers(0, 1 << 63, n, dtype=np.uint64),
+            rng.random(n).astype(np.float32),
+            rng.standard_normal(n),
+            rng.integers(-100, 100, n).astype(np.int8)]
+    reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MIN), (0, A.MAX), (1, A.SUM), (1, A.MIN), (1, A.MAX),
+            (2, A.SUM), (2, A.MAX), (3, A.MIN), (4, A.SUM), (4, A.MAX), (5, A.MIN), (5, A.MAX),
+            (5, A.AVG), (1, A.AVG), (6, A.SUM), (6, A.MAX)]
+    return keys, vals, reqs
+
+
+@pytest.mark.parametrize("name,fn", list(STRATEGIES.items()))
+@pytest.mark.parametrize("kt", [np.int32, np.int64, np.uint32, np.uint64])
+@pytest.mark.parametrize("ngroups", [5, 3000, 60000])
+def test_all_kinds_and_types_match_oracle(name, fn, kt, ngroups):
+    keys, vals, reqs = _rand_case(ngroups, kt, 200_000, ngroups)
+    if name != "auto":
+        reqs = [r for r in reqs if r[1] != A.AVG]
+    got = sorted_by_key(run(fn, keys, vals, reqs))
+    exp = O.groupby(keys, vals, reqs)
+    # double sums/averages: fp64 accumulation in a different order
+    assert_same(got, exp, float_rtol=1e-9)
+
+
+@pytest.mark.parametrize("name,fn", list(STRATEGIES.items()))
+def test_any_returns_a_member_of_the_group(name, fn):
+    rng = np.random.default_rng(1)
+    keys = rng.integers(0, 100, 10_000).astype(np.int32)
+    vals = rng.integers(0, 1 << 30, 10_000).astype(np.int64)
+    got = sorted_by_key(run(fn, keys, [vals], [(0, A.ANY)]))
+    members = {}
+    for k, v in zip(keys.tolist(), vals.tolist()):
+        members.setdefault(k, set()).add(v)
+    for k, v in zip(got[0].tolist(), got[1].tolist()):
+        assert v in members[k]
+
+
+@pytest.mark.parametrize("name,fn", list(STRATEGIES.items()))
+def test_extreme_keys_including_the_empty_encoding(name, fn):
+    """INT64_MIN encodes to the empty word and takes the side slot; INT32_MIN
+    takes the LDS side slot of 4-byte keys."""
+    for kt in (np.int64, np.int32, np.uint64, np.uint32):
+        info = np.iinfo(kt)
+        special = np.array([info.min, info.max, 0, 1, info.max - 1, (info.max // 2) + 1], dtype=kt)
+        keys = np.tile(special, 1000)
+        np.random.default_rng(0).shuffle(keys)
+        vals = np.arange(len(keys), dtype=np.int64)
+        reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MIN), (0, A.MAX)]
+        got = sorted_by_key(run(fn, keys, [vals], reqs))
+        assert_same(got, O.groupby(keys, [vals], reqs))
+
+
+@pytest.mark.parametrize("name,fn", list(STRATEGIES.items()))
+def test_empty_input_returns_zero_rows_with_schema(name, fn):
+    """phys/phys.cu:589-611 (phys_test.cu:467-476)."""
+    keys = np.empty(0, np.int32)
+    vals = [np.empty(0, np.float64)]
+    out = run(fn, keys, vals, [(0, A.SUM)])
+    assert len(out) == 2 and len(out[0]) == 0 and out[0].dtype == np.int32 and out[1].dtype == np.float64
+
+
+@pytest.mark.parametrize("name,fn", list(STRATEGIES.items()))
+def test_single_row_and_single_group(name, fn):
+    for n in (1, 2, 1000, 100_000):
+        keys = np.full(n, -7, np.int32)
+        vals = np.arange(n, dtype=np.int64)
+        reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MAX)]
+        assert_same(sorted_by_key(run(fn, keys, [vals], reqs)), O.groupby(keys, [vals], reqs))
+
+
+@pytest.mark.parametrize("name,fn", list(STRATEGIES.items()))
+def test_groups_beyond_the_lds_table_spill_to_hbm(name, fn):
+    n = 1 << 20
+    keys = O.gen_keys_uniform(n, 300_000, seed=5, dtype=np.int64)
+    vals = O.gen_values(n, O.VAL_UNIFORM_I64, np.int64, seed=5, stream_id=2)
+    reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MIN), (0, A.MAX)]
+    got = sorted_by_key(run(fn, keys, [vals], reqs, G.groupby_hint(groups_upper_bound=300_000)))
+    assert_same(got, O.groupby(keys, [vals], reqs))
+
+
+def test_hash_table_overflow_retries_with_a_bigger_table():
+    """A groups_upper_bound far below the truth fills the HBM table; the
+    operator retries instead of returning a partial result."""
+    n = 1 << 18
+    keys = np.arange(n, dtype=np.int32)[::-1].copy()
+    vals = np.ones(n, dtype=np.int64)
+    reqs = [(-1, A.COUNT_ALL), (0, A.SUM)]
+    got = sorted_by_key(run(G.hash_groupby, keys, [vals], reqs, G.groupby_hint(groups_upper_bound=10)))
+    assert_same(got, O.groupby(keys, [vals], reqs))
+
+
+def test_float32_sum_within_1e6_of_fp64_oracle():
+    n = 1 << 24
+    keys = O.gen_keys_uniform(n, 16, seed=42)
+    vf = O.gen_values(n, O.VAL_UNIFORM_F32, np.float32, seed=42, stream_id=3)
+    got = sorted_by_key(run(G.hash_groupby, keys, [vf], [(0, A.SUM)]))
+    exp = O.groupby(keys, [vf], [(0, A.SUM)])
+    np.testing.assert_array_equal(got[0], exp[0])
+    rel = np.abs(got[1].astype(np.float64) - exp[1].astype(np.float64)) / np.abs(exp[1].astype(np.float64))
+    assert rel.max() <= 1e-6
+
+
+# ------------------------------------------------------------------ dispatch / stats
+@pytest.mark.parametrize("case", FIX["ndv"], ids=lambda c: c["name"])
+def test_ndv_estimate_bounds(case):
+    """stats/stats_compute_test.cu:197-244."""
+    c = G.make_column_from_host(np.array(case["data"], np.int32))
+    est = G.compute_ndv_approx(c)
+    assert case["lo"] <= est <= case["hi"]
+
+
+def test_ndv_estimate_accuracy_at_scale():
+    for d in (1000, 1 << 16, 1 << 20):
+        c = G.gen_keys_uniform(1 << 22, d, seed=1)
+        est = G.compute_ndv_approx(c)
+        assert abs(est - d) / d < 0.04, (d, est)
+
+
+def test_auto_resolves_the_hint_in_place():
+    """groupby.cu:68-119: AUTO becomes HASH for <= 2^14 estimated groups."""
+    keys = O.gen_keys_uniform(1 << 16, 100, seed=2)
+    h = G.groupby_hint()
+    G.auto_select_groupby_algo([G.make_column_from_host(keys)], h)
+    assert h.groupby_algo == G.groupby_algo.HASH
+    h = G.groupby_hint()
+    run(G.groupby, keys, [keys], [(0, A.SUM)], h)
+    assert h.groupby_algo == G.groupby_algo.HASH
+
+
+# ------------------------------------------------------------------ scalar aggregate
+@pytest.mark.parametrize("case", FIX["scalar"], ids=lambda c: c["name"])
+def test_scalar_aggregate_closed_forms(case):
+    """tests/aggregate/aggregate_test.cu:11-111."""
+    t = np.dtype(case["type"])
+    c0 = G.gen_values(case["rows"], G.val_dist.SEQ, G.to_type_id(t))
+    c1 = G.gen_values(case["rows"], G.val_dist.SEQ, G.to_type_id(t))
+    tab = G.aggregate([c0, c1], [(0, case["kind"]), (1, case["kind"])])
+    assert tab.num_columns() == 2 and tab.num_rows() == 1
+    out = tab.to_numpy()
+    assert int(out[0][0]) == case["expected"] and int(out[1][0]) == case["expected"]
+
+
+def test_scalar_aggregate_matches_oracle():
+    rng = np.random.default_rng(4)
+    n = 300_001
+    cols = [rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64), rng.random(n).astype(np.float32),
+            rng.integers(0, 1 << 31, n).astype(np.uint32)]
+    reqs = [(0, A.SUM), (0, A.MIN), (0, A.MAX), (1, A.SUM), (1, A.MAX), (1, A.AVG), (2, A.SUM),
+            (2, A.MIN), (-1, A.COUNT_ALL), (0, A.AVG)]
+    tab = G.aggregate([G.make_column_from_host(c) for c in cols], reqs)
+    assert_same(tab.to_numpy(), O.aggregate(cols, reqs), float_rtol=1e-6)
+
+
+def test_scalar_aggregate_empty_input_identities():
+    """Empty input keeps the reference functors' identities (agg_func.cuh:37-207)."""
+    c = G.make_column_from_host(np.empty(0, np.int32))
+    out = G.aggregate([c], [(0, A.SUM), (0, A.MAX), (0, A.MIN), (-1, A.COUNT_ALL)]).to_numpy()
+    assert [int(o[0]) for o in out] == [0, np.iinfo(np.int32).min, np.iinfo(np.int32).max, 0]
+
+
+# ------------------------------------------------------------------ full-size properties
+@pytest.mark.parametrize("G_,zipf", [(4, 0.0), (1024, 0.0), (1 << 20, 0.0), (1024, 1.0), (1 << 22, 1.0)])
+def test_reference_closed_form_suite(G_, zipf):
+    """groupby_test.hpp:64-92,175-218 regenerated: N = 2^28 rows, values all
+    ones, two SUM requests, hint.groups_upper_bound = G.  Uniform: every SUM is
+    N/G and the keys sum to G(G-1)/2.  Zipf: SUM equals the key's count."""
+    n = 1 << 28
+    if zipf == 0.0:
+        k = G.gen_keys_uniform(n, G_, seed=42)
+    else:
+        k = G.gen_keys_zipf(n, G_, zipf, seed=42)
+    ones0 = G.gen_values(n, G.val_dist.ALL_ONES, G.type_id.INT32)
+    ones1 = G.gen_values(n, G.val_dist.ALL_ONES, G.type_id.INT32)
+    t = G.groupby([k], [ones0, ones1], [(0, A.SUM), (1, A.SUM)], G.groupby_hint(groups_upper_bound=G_))
+    keys, s0, s1 = t.to_numpy()
+    t.free()
+    np.testing.assert_array_equal(s0, s1)
+    if zipf == 0.0:
+        assert len(keys) == G_
+        assert int(keys.astype(np.int64).sum()) == G_ * (G_ - 1) // 2
+        assert np.all(s0 == n // G_)
+    else:
+        hk = k.to_numpy()
+        counts = np.bincount(hk, minlength=G_)
+        assert len(keys) == int(np.count_nonzero(counts))
+        np.testing.assert_array_equal(s0, counts[keys])
+    for c in (k, ones0, ones1):
+        c.free()
+
+
+def test_full_size_sweep_shape_invariants():
+    """The benchmark workload shape (2^28 rows, COUNT, SUM/MIN/MAX(int64),
+    SUM(float32)): counts sum to N, integer sums sum to the column total,
+    MIN <= MAX, and the float sums to the column total within 1e-6."""
+    n = 1 << 28
+    d = 1 << 16
+    k = G.gen_keys_uniform(n, d, seed=42)
+    v = G.gen_values(n, G.val_dist.UNIFORM_I64, G.type_id.INT64, seed=42, stream_id=2)
+    f = G.gen_values(n, G.val_dist.UNIFORM_F32, G.type_id.FLOAT32, seed=42, stream_id=3)
+    reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MIN), (0, A.MAX), (1, A.SUM)]
+    t = G.groupby([k], [v, f], reqs, G.groupby_hint(groups_upper_bound=d))
+    keys, cnt, s, mn, mx, fs = t.to_numpy()
+    t.free()
+    tot = G.aggregate([v, f], [(0, A.SUM), (1, A.SUM)]).to_numpy()
+    assert len(keys) == d and len(np.unique(keys)) == d
+    assert int(cnt.sum()) == n and np.all(cnt == n // d)
+    assert int(s.astype(np.int64).sum()) == int(tot[0][0])
+    assert np.all(mn <= mx)
+    assert abs(float(fs.astype(np.float64).sum()) - float(tot[1][0])) / float(tot[1][0]) < 1e-6
