@@ -116,6 +116,10 @@ struct DeviceInfo {
 };
 const DeviceInfo& device_info();
 
+// Raises the kernel's dynamic LDS limit to what its static LDS leaves of the
+// workgroup's LDS; returns that many bytes.  Cached per kernel.
+size_t enable_dynamic_lds(const void* kernel);
+
 // ---- kernel timing -----------------------------------------------------------
 bool timing_enabled();
 // Records hipEvents around a launch on `s` when timing is enabled.

@@ -1,0 +1,229 @@
+// eg_lds_table.hpp -- building blocks of the LDS-privatised aggregation used
+// by the hash and partition strategies: compiled column layouts with vector
+// loads, the replicated open-addressing LDS table, canonical value
+// conversion, and the per-row LDS / HBM update paths (with wave-level
+// combining of hot groups).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "eg_atomics.hpp"
+#include "eg_device.hpp"
+#include "eg_hash_table.hpp"
+#include "eg_internal.hpp"
+
+namespace eg {
+namespace lds {
+
+constexpr int kRows = 4;      // consecutive rows per lane per tile
+
+template <int KB>
+struct LdsKey {
+    using T = typename std::conditional<KB == 4, uint32_t, unsigned long long>::type;
+};
+
+// Four consecutive elements (row r multiple of 4, 16-byte aligned column).
+template <int W>
+__device__ __forceinline__ void vload4(const void* p, int64_t r, uint64_t (&o)[kRows])
+{
+    if constexpr (W == 4) {
+        uint4 v = *(const uint4*)((const uint32_t*)p + r);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    } else {
+        const uint4* q = (const uint4*)((const uint64_t*)p + r);
+        uint4 a = q[0], b = q[1];
+        o[0] = a.x | ((uint64_t)a.y << 32); o[1] = a.z | ((uint64_t)a.w << 32);
+        o[2] = b.x | ((uint64_t)b.y << 32); o[3] = b.z | ((uint64_t)b.w << 32);
+    }
+}
+
+__device__ __forceinline__ uint64_t pick4(const uint64_t (&a)[kRows], int i)
+{
+    return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3];
+}
+
+template <int NV>
+__device__ __forceinline__ uint64_t pick_col(const uint64_t (&v)[NV], int c)
+{
+    uint64_t x = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        if (k == c) x = v[k];
+    return x;
+}
+
+// Value-column layouts compiled into the kernel.  GEN: any widths, read with
+// per-element typed loads; otherwise NV columns of W0/W1 bytes read with
+// 16-byte vector loads.
+template <int NV_, int W0_, int W1_, bool GEN_, int C0_ = CONV_RAW, int C1_ = CONV_RAW>
+struct VLayout {
+    static constexpr int NV = NV_;
+    static constexpr bool GEN = GEN_;
+    template <int C>
+    static constexpr int width() { return C == 0 ? W0_ : W1_; }
+    template <int C>
+    static constexpr int conv() { return C == 0 ? C0_ : C1_; }
+};
+
+template <class VL>
+__device__ __forceinline__ void load_tile(const KeyDesc& key, int kb, const AggSpec& spec, int64_t r0,
+                                          uint64_t (&k)[kRows], uint64_t (&v)[VL::NV > 0 ? VL::NV : 1][kRows])
+{
+    if constexpr (VL::GEN) {
+#pragma unroll
+        for (int i = 0; i < kRows; ++i) k[i] = dev::load_raw(key.data, kb, r0 + i);
+#pragma unroll
+        for (int c = 0; c < VL::NV; ++c)
+            if (c < spec.nvcols)
+#pragma unroll
+                for (int i = 0; i < kRows; ++i) v[c][i] = dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, r0 + i);
+    } else {
+        if (kb == 4) vload4<4>(key.data, r0, k);
+        else vload4<8>(key.data, r0, k);
+        if constexpr (VL::NV >= 1) vload4<VL::template width<0>()>(spec.cols[0].data, r0, v[0]);
+        if constexpr (VL::NV >= 2) vload4<VL::template width<1>()>(spec.cols[1].data, r0, v[1]);
+    }
+}
+
+// Layout of the dynamic LDS (8-byte words), T = (Cr + 1) * R slots:
+//   states: nops * T words; op a of slot idx at [a * T + idx]
+//   keys:   T entries of KB bytes (encoded; 0 = empty)
+//   side:   R words, side slot of replica r used
+// Slot (s, r) has index (s << rlog) | r; s = Cr is the side slot holding the
+// key whose LDS encoding is zero.
+template <int KB>
+struct LdsTable {
+    using KT = typename LdsKey<KB>::T;
+    unsigned long long* st;
+    KT* keys;
+    unsigned long long* side;
+    int T, cr, rlog;
+
+    __device__ LdsTable(unsigned long long* smem, int nops, int cr_log2, int rep_log2)
+    {
+        cr = 1 << cr_log2;
+        rlog = rep_log2;
+        T = (cr + 1) << rlog;
+        st = smem;
+        keys = reinterpret_cast<KT*>(st + (size_t)nops * T);
+        side = st + (size_t)nops * T + ((size_t)T * KB + 7) / 8;
+    }
+    __device__ static int words(int nops, int cr_log2, int rep_log2)
+    {
+        const int T = ((1 << cr_log2) + 1) << rep_log2;
+        return nops * T + (T * KB + 7) / 8 + (1 << rep_log2);
+    }
+    // Finds or inserts encoded key e in replica r; -1 when probe_limit slots
+    // are taken by other keys.
+    __device__ __forceinline__ int find(KT e, int r, int probe_limit)
+    {
+        if (e == 0) {
+            if (__hip_atomic_load(&side[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0ull)
+                __hip_atomic_store(&side[r], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return (cr << rlog) | r;
+        }
+        uint32_t h = (KB == 4) ? dev::fmix32((uint32_t)e) : (uint32_t)(dev::fmix64((uint64_t)e) >> 32);
+        h &= (uint32_t)(cr - 1);
+        for (int p = 0; p < probe_limit; ++p) {
+            const int idx = (int)(h << rlog) | r;
+            KT cur = __hip_atomic_load(&keys[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (cur == e) return idx;
+            if (cur == 0) {
+                KT prev = atomicCAS(&keys[idx], (KT)0, e);
+                if (prev == 0 || prev == e) return idx;
+            }
+            h = (h + 1) & (uint32_t)(cr - 1);
+        }
+        return -1;
+    }
+};
+
+template <int KB>
+__device__ __forceinline__ uint64_t lds_to_enc64(typename LdsKey<KB>::T kw, int is_signed)
+{
+    if (KB == 4) return dev::enc64(dev::widen_key((uint32_t)kw ^ dev::kSign32, 4, is_signed));
+    return (uint64_t)kw;
+}
+
+// Canonical values of a row's columns (raw -> 64-bit per ConvCode); the
+// compiled layouts know each column's conversion.
+template <class VL, int NV>
+__device__ __forceinline__ void canon_row(const AggSpec& spec, const uint64_t (&raw)[NV], uint64_t (&cv)[NV])
+{
+    if constexpr (VL::GEN) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) cv[c] = c < spec.nvcols ? dev::canon(raw[c], spec.cols[c].conv) : 0ull;
+    } else {
+        if constexpr (VL::NV >= 1) cv[0] = dev::canon_c<VL::template conv<0>()>(raw[0]);
+        if constexpr (VL::NV >= 2) cv[1] = dev::canon_c<VL::template conv<1>()>(raw[1]);
+        if constexpr (VL::NV == 0) cv[0] = 0ull;
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ uint64_t operand_of(const AggSpec& spec, int a, const uint64_t (&cv)[NV])
+{
+    const OpDesc& od = spec.ops[a];
+    return dev::operand(od, od.col < 0 ? 1ull : pick_col<NV>(cv, od.col));
+}
+
+// One row into the LDS table; returns true when it needs the HBM table.
+template <int KB, int NV>
+__device__ __forceinline__ bool lds_row(LdsTable<KB>& tab, bool use_lds, int r_mask, int lane, const KeyDesc& key,
+                                        const AggSpec& spec, uint64_t kraw, const uint64_t (&cv)[NV],
+                                        bool valid, int probe_limit, int peel_min)
+{
+    using KT = typename LdsKey<KB>::T;
+    int slot = -1;
+    if (use_lds) {
+        // first probe inline: once the table holds the groups it almost always hits
+        KT e = 0;
+        uint32_t h = 0;
+        bool hit = false;
+        if (valid) {
+            e = (KB == 4) ? (KT)((uint32_t)kraw ^ dev::kSign32) : (KT)dev::enc64(dev::widen_key(kraw, KB, key.is_signed));
+            h = ((KB == 4) ? dev::fmix32((uint32_t)e) : (uint32_t)(dev::fmix64((uint64_t)e) >> 32)) & (uint32_t)(tab.cr - 1);
+            const int idx = (int)(h << tab.rlog) | (lane & r_mask);
+            const KT cur = __hip_atomic_load(&tab.keys[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            hit = e != 0 && cur == e;
+            if (hit) slot = idx;
+        }
+        if (__ballot(valid && !hit) != 0ull) {
+            if (valid && !hit) slot = tab.find(e, lane & r_mask, probe_limit);
+        }
+    }
+    bool done = false;
+    if (use_lds && r_mask == 0 && peel_min <= 64) {
+        done = dev::wave_peel(slot >= 0, (uint64_t)(int64_t)slot, spec, peel_min,
+                              [&](int a) { return operand_of<NV>(spec, a, cv); },
+                              [&](uint64_t s) { return &tab.st[(int64_t)s]; }, (int64_t)tab.T);
+    }
+    if (slot >= 0 && !done) dev::apply_ops<NV>(spec, cv, [&](int a) { return &tab.st[(size_t)a * tab.T + slot]; });
+    return valid && slot < 0;
+}
+
+// One row into the HBM table (called by every lane of the wave).
+template <int KB, int NV>
+__device__ __forceinline__ void hbm_row(const GTable& gt, const KeyDesc& key, const AggSpec& spec, bool pending,
+                                        uint64_t kraw, const uint64_t (&cv)[NV], int peel_min)
+{
+    const uint64_t e64 = dev::enc64(dev::widen_key(kraw, KB, key.is_signed));
+    bool done = false;
+    if (peel_min <= 64)
+        done = dev::wave_peel(pending, e64, spec, peel_min, [&](int a) { return operand_of<NV>(spec, a, cv); },
+                              [&](uint64_t t) {
+                                  unsigned long long* g = gt_slot(gt, t);
+                                  return g ? g + 1 : (unsigned long long*)nullptr;
+                              },
+                              (int64_t)1);
+    if (pending && !done) {
+        unsigned long long* g = gt_slot(gt, e64);
+        if (g) dev::apply_ops<NV>(spec, cv, [&](int a) { return g + 1 + a; });
+    }
+}
+
+
+}  // namespace lds
+}  // namespace eg

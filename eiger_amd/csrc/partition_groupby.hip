@@ -1,0 +1,342 @@
+// partition_groupby.hip -- partition-based group-by
+// (gamax::groupby::partition::groupby / groupby_gfur,
+// operators/groupby/groupby_partition.cu:556-707).
+//
+// The reference computes pid = fmix(key) & 0xFFFF, sorts (pid, key) and every
+// value column with a 16-bit CUB radix sort each (groupby_partition.cu:580-663),
+// then aggregates one partition per block in a shared-memory cuco set of 8,209
+// row indices and throws when a partition has more distinct keys
+// (:116-297, 543-549).  Here:
+//   1. the number of partitions P = 2^b is chosen so that a partition holds
+//      about a thousand groups (from hint.groups_upper_bound or the HyperLogLog
+//      estimate);
+//   2. LSD radix passes on the fmix bits move the key and every value column
+//      together (PARTITION) or the key and row ids (PARTITION_GFUR, values
+//      gathered at aggregation time);
+//   3. part_bounds_kernel finds each partition's first row;
+//   4. part_agg_kernel aggregates a unit (a partition, or a slice of a large
+//      one) in an LDS table and writes the finished groups of single-unit
+//      partitions straight to the output.  Groups of sliced partitions, and
+//      keys that do not fit the LDS table, go through the HBM table and are
+//      appended by its compaction -- a spill, where the reference throws.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "eg_atomics.hpp"
+#include "eg_device.hpp"
+#include "eg_hash_table.hpp"
+#include "eg_internal.hpp"
+#include "eg_lds_table.hpp"
+#include "eg_radix.hpp"
+
+namespace eg {
+namespace {
+
+using namespace lds;
+
+constexpr int kPB = 1024;  // threads per unit block
+
+struct Unit {
+    int64_t start;
+    int64_t end;
+    int32_t sliced;  // the partition spans several units
+    int32_t pad;
+};
+
+template <int KB>
+__device__ __forceinline__ uint32_t pid_of(uint64_t raw, uint32_t mask)
+{
+    return (uint32_t)dev::bit_mix<KB>(raw) & mask;
+}
+
+template <int KB>
+__global__ void part_bounds_kernel(const void* keys, int64_t n, uint32_t mask, int64_t* offs)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t p = pid_of<KB>(dev::load_raw(keys, KB, i), mask);
+        const int64_t pp = i == 0 ? -1 : (int64_t)pid_of<KB>(dev::load_raw(keys, KB, i - 1), mask);
+        for (int64_t q = pp + 1; q <= p; ++q) offs[q] = i;
+        if (i == n - 1)
+            for (int64_t q = p + 1; q <= (int64_t)mask + 1; ++q) offs[q] = n;
+    }
+}
+
+template <int KB, int NV>
+__global__ __launch_bounds__(kPB) void part_agg_kernel(KeyDesc key, AggSpec spec, const int64_t* perm,
+                                                       const Unit* units, GTable gt, OutSpec os,
+                                                       unsigned long long* counter, int64_t capacity,
+                                                       int cr_log2, int probe_limit, int peel_min)
+{
+    using KT = typename LdsKey<KB>::T;
+    extern __shared__ unsigned long long smem[];
+    const int nops = spec.nops;
+    const int lane = dev::lane_id();
+    LdsTable<KB> tab(smem, nops, cr_log2, 0);
+    const int words = LdsTable<KB>::words(nops, cr_log2, 0);
+    for (int i = threadIdx.x; i < words; i += blockDim.x) smem[i] = 0ull;
+    __syncthreads();
+
+    const Unit u = units[blockIdx.x];
+    for (int64_t b = u.start; b < u.end; b += blockDim.x) {
+        const int64_t r = b + threadIdx.x;
+        const bool valid = r < u.end;
+        const int64_t src = (perm && valid) ? perm[r] : r;
+        const uint64_t kraw = valid ? dev::load_raw(key.data, KB, r) : 0ull;
+        uint64_t raw[NV], cv[NV];
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+            raw[c] = (valid && c < spec.nvcols) ? dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, src) : 0ull;
+#pragma unroll
+        for (int c = 0; c < NV; ++c) cv[c] = c < spec.nvcols ? dev::canon(raw[c], spec.cols[c].conv) : 0ull;
+        const bool sp = lds_row<KB, NV>(tab, true, 0, lane, key, spec, kraw, cv, valid, probe_limit, peel_min);
+        if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kraw, cv, peel_min);
+    }
+    __syncthreads();
+    // finished groups of this unit
+    for (int base = 0; base <= tab.cr; base += blockDim.x) {
+        const int s = base + threadIdx.x;
+        bool occ = false;
+        KT kw = 0;
+        if (s < tab.cr) {
+            kw = tab.keys[s];
+            occ = kw != 0;
+        } else if (s == tab.cr) {
+            occ = tab.side[0] != 0ull;
+        }
+        const uint64_t enc = lds_to_enc64<KB>(kw, key.is_signed);
+        if (u.sliced) {
+            if (occ) {
+                unsigned long long* g = gt_slot(gt, enc);
+                if (g)
+                    dev::merge_states(spec, [&](int a) { return g + 1 + a; },
+                                      [&](int a) { return tab.st[(size_t)a * tab.T + s]; });
+            }
+        } else {
+            const int64_t idx = dev::wave_claim(counter, occ);
+            if (occ) {
+                if (idx < capacity) dev::emit_group(os, idx, enc, tab.st + s, (int64_t)tab.T);
+                else atomicExch(&gt.meta[2], 1ull);  // output capacity exceeded: retry larger
+            }
+        }
+    }
+}
+
+// Appends the HBM table's groups after the directly written ones.
+__global__ void part_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter, int64_t capacity)
+{
+    const uint64_t total = gt.mask + 2;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
+        const uint64_t s = base + threadIdx.x;
+        bool occ = false;
+        const unsigned long long* slot = nullptr;
+        uint64_t enc = 0;
+        if (s <= gt.mask) {
+            slot = gt.base + s * (uint64_t)gt.W;
+            enc = slot[0];
+            occ = enc != 0;
+        } else if (s == gt.mask + 1) {
+            slot = gt.side;
+            occ = gt.meta[1] != 0;
+        }
+        const int64_t idx = dev::wave_claim(counter, occ);
+        if (occ && idx < capacity) dev::emit_group(os, idx, enc, slot + 1, 1);
+    }
+}
+
+int ceil_log2(int64_t x)
+{
+    int b = 0;
+    while (((int64_t)1 << b) < x) ++b;
+    return b;
+}
+
+template <int KB>
+void launch_part_agg(const KeyDesc& key, const AggSpec& spec, const int64_t* perm, const Unit* units, int nunits,
+                     const GTable& gt, const OutSpec& os, unsigned long long* counter, int64_t capacity, int cr_log2,
+                     size_t lds, hipStream_t s)
+{
+    const int nv = spec.nvcols;
+    auto go = [&](auto kern) {
+        if (enable_dynamic_lds((const void*)kern) < lds) raise(EG_ERR_UNSUPPORTED, "partition LDS table too large");
+        KernelTimer timer("partition_agg", s);
+        hipLaunchKernelGGL(kern, dim3(nunits), dim3(kPB), lds, s, key, spec, perm, units, gt, os, counter, capacity,
+                           cr_log2, 16, 8);
+        EG_LAUNCH_CHECK();
+    };
+    if (nv <= 1) go(part_agg_kernel<KB, 1>);
+    else if (nv <= 2) go(part_agg_kernel<KB, 2>);
+    else if (nv <= 4) go(part_agg_kernel<KB, 4>);
+    else go(part_agg_kernel<KB, 8>);
+}
+
+}  // namespace
+
+Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream_t s)
+{
+    const DeviceInfo& di = device_info();
+    const int64_t n = plan.rows;
+    const int kb = plan.key.bytes;
+    const int nops = plan.spec.nops;
+    const int64_t bound = std::max<int64_t>(1, std::min<int64_t>(group_bound, n));
+
+    // LDS table: the largest power of two <= 2048 slots that fits one block
+    int cr_log2 = 11;
+    auto lds_of = [&](int c) {
+        const size_t T = (size_t)(1 << c) + 1;
+        return (size_t)nops * T * 8 + (T * kb + 7) / 8 * 8 + 8;
+    };
+    while (cr_log2 > 6 && lds_of(cr_log2) + 256 > (size_t)di.max_lds) --cr_log2;  // 256: static LDS
+    const size_t lds = lds_of(cr_log2);
+    const int64_t per_part = (int64_t)1 << (cr_log2 - 1);  // target groups per partition (load 1/2)
+    const int bits = std::min(24, std::max(1, ceil_log2((bound + per_part - 1) / per_part)));
+    const uint32_t pmask = (uint32_t)((1u << bits) - 1u);
+
+    // ---- radix passes on the hash bits -----------------------------------------
+    RadixKeys rk{};
+    rk.data = plan.key.data;
+    rk.bytes = kb;
+    rk.type = plan.key_type;
+    rk.mode = DIG_HASH;
+    SortBuffers b;
+    b.keys_in = plan.key.data;
+    DevBuf ka((size_t)n * kb, s), kbuf((size_t)n * kb, s);
+    b.keys_a = ka.get();
+    b.keys_b = kbuf.get();
+    std::vector<DevBuf> bufs;
+    AggSpec spec = plan.spec;
+    if (gfur) {
+        b.pay_in.push_back(Payload{nullptr, nullptr, 8, 1});
+        bufs.emplace_back((size_t)n * 8, s);
+        bufs.emplace_back((size_t)n * 8, s);
+        b.pay_a.push_back(bufs[0].get());
+        b.pay_b.push_back(bufs[1].get());
+    } else {
+        for (int c = 0; c < spec.nvcols; ++c) {
+            const int by = spec.cols[c].bytes;
+            b.pay_in.push_back(Payload{spec.cols[c].data, nullptr, by, 0});
+            bufs.emplace_back((size_t)n * by, s);
+            bufs.emplace_back((size_t)n * by, s);
+            b.pay_a.push_back(bufs[bufs.size() - 2].get());
+            b.pay_b.push_back(bufs[bufs.size() - 1].get());
+        }
+    }
+    // balanced digits: ceil(bits / 8) passes of near-equal width
+    const int passes = (bits + 7) / 8;
+    const void* pkeys = nullptr;
+    std::vector<const void*> ppay;
+    {
+        // radix_sort walks [0, bits) in 8-bit steps; balance by sorting the
+        // low passes first with equal widths
+        const int w = (bits + passes - 1) / passes;
+        const void* kin = b.keys_in;
+        std::vector<Payload> pay = b.pay_in;
+        bool to_a = true;
+        for (int shift = 0; shift < bits; shift += w) {
+            const int bw = std::min(w, bits - shift);
+            void* kout = to_a ? b.keys_a : b.keys_b;
+            for (size_t c = 0; c < pay.size(); ++c) pay[c].out = to_a ? b.pay_a[c] : b.pay_b[c];
+            RadixKeys k = rk;
+            k.data = kin;
+            radix_pass(k, kout, pay, n, shift, bw, s);
+            kin = kout;
+            for (auto& p : pay) {
+                p.in = p.out;
+                if (p.rowid) {
+                    p.rowid = 0;
+                    p.bytes = 8;
+                }
+            }
+            to_a = !to_a;
+        }
+        pkeys = kin;
+        for (auto& p : pay) ppay.push_back(p.in);
+    }
+    const int64_t* perm = nullptr;
+    if (gfur) perm = (const int64_t*)ppay[0];
+    else
+        for (int c = 0; c < spec.nvcols; ++c) spec.cols[c].data = ppay[(size_t)c];
+    KeyDesc pkey = plan.key;
+    pkey.data = pkeys;
+
+    // ---- partition boundaries and units -------------------------------------------
+    const int64_t P = (int64_t)pmask + 1;
+    DevBuf offs((size_t)(P + 1) * 8, s);
+    {
+        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)di.num_cus * 8));
+        KernelTimer timer("partition_bounds", s);
+        if (kb == 4) hipLaunchKernelGGL(part_bounds_kernel<4>, dim3((unsigned)grid), dim3(256), 0, s, pkeys, n, pmask, offs.get<int64_t>());
+        else hipLaunchKernelGGL(part_bounds_kernel<8>, dim3((unsigned)grid), dim3(256), 0, s, pkeys, n, pmask, offs.get<int64_t>());
+        EG_LAUNCH_CHECK();
+    }
+    std::vector<int64_t> ho((size_t)(P + 1));
+    EG_HIP(hipMemcpyAsync(ho.data(), offs.get(), (size_t)(P + 1) * 8, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    const int64_t slice = std::max<int64_t>((int64_t)kPB * 16, 8 * per_part);  // rows per unit of a large partition
+    std::vector<Unit> units;
+    for (int64_t p = 0; p < P; ++p) {
+        const int64_t a = ho[(size_t)p], e = ho[(size_t)p + 1];
+        if (e <= a) continue;
+        if (e - a <= slice) {
+            units.push_back(Unit{a, e, 0, 0});
+        } else {
+            for (int64_t x = a; x < e; x += slice) units.push_back(Unit{x, std::min(e, x + slice), 1, 0});
+        }
+    }
+    if (units.size() >= ((size_t)1 << 31)) raise(EG_ERR_UNSUPPORTED, "too many partition units");
+    DevBuf dunits(units.size() * sizeof(Unit), s);
+    EG_HIP(hipMemcpyAsync(dunits.get(), units.data(), units.size() * sizeof(Unit), hipMemcpyHostToDevice, s));
+
+    // ---- aggregation ------------------------------------------------------------------
+    int64_t capacity = bound;
+    int64_t slots = 1 << 14;
+    const int W = table_words_per_slot(nops);
+    for (;;) {
+        Result res;
+        OutSpec os;
+        alloc_outputs(plan, capacity, s, res, os);
+        HbmTable ht(slots, W, s);
+        DevBuf counter(8, s);
+        fill_zero(counter.get(), 8, s);
+        if (kb == 4)
+            launch_part_agg<4>(pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
+                               counter.get<unsigned long long>(), capacity, cr_log2, lds, s);
+        else
+            launch_part_agg<8>(pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
+                               counter.get<unsigned long long>(), capacity, cr_log2, lds, s);
+        int64_t meta[3];
+        ht.read_meta(meta, s);
+        const bool table_full = meta[2] != 0;
+        if (!table_full) {
+            // append HBM-table groups
+            const uint64_t total = ht.gt.mask + 2;
+            const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + 255) / 256), (int64_t)di.num_cus * 8));
+            hipLaunchKernelGGL(part_compact_kernel, dim3((unsigned)grid), dim3(256), 0, s, ht.gt, os,
+                               counter.get<unsigned long long>(), capacity);
+            EG_LAUNCH_CHECK();
+        }
+        int64_t* h = pinned_i64(1);
+        EG_HIP(hipMemcpyAsync(h, counter.get(), 8, hipMemcpyDeviceToHost, s));
+        EG_HIP(hipStreamSynchronize(s));
+        const int64_t groups = h[0];
+        if (table_full || groups > capacity) {
+            // the HBM table or the output ran out: retry with room for every row
+            if (capacity >= n && slots >= 2 * n) raise(EG_ERR_CAPACITY, "partition group-by overflow");
+            if (groups > capacity) capacity = n;
+            if (table_full) {
+                int64_t ns = slots;
+                while (ns < 2 * std::max<int64_t>(bound, meta[0] * 2)) ns <<= 1;
+                slots = std::max<int64_t>(ns, slots * 4);
+            }
+            continue;
+        }
+        res.num_groups = groups;
+        return res;
+    }
+}
+
+}  // namespace eg

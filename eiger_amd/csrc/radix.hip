@@ -1,0 +1,386 @@
+// radix.hip -- stable radix passes, LSD sort and the sort/partition
+// sub-operators (gamax::sort, gamax::partition).
+//
+// The reference delegates these to cub::DeviceRadixSort / DeviceHistogram /
+// DeviceScan (operators/sort/sort.cu:15-66, operators/partition/partition.cu:
+// 24-171).  Here one pass over 8 digit bits is:
+//   radix_hist_kernel    per 4096-row tile, an LDS histogram of the digit;
+//   scan                 exclusive prefix over [digit][tile] (device-wide);
+//   radix_scatter_kernel per tile, a stable rank of every row among the
+//                        tile's rows of the same digit (wave64 ballot match on
+//                        the digit bits, then a prefix over the 4 waves and
+//                        the 16 rounds), the tile rearranged in LDS so rows of
+//                        one digit are contiguous, then written out in runs
+//                        (coalesced stores).  Payload columns follow the same
+//                        permutation, one column at a time through LDS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "eg_device.hpp"
+#include "eg_internal.hpp"
+#include "eg_radix.hpp"
+
+namespace eg {
+namespace {
+
+constexpr int kRB = 256;        // threads per block
+constexpr int kRI = 16;         // rows per thread
+constexpr int kRT = kRB * kRI;  // rows per tile
+
+struct PayArgs {
+    int32_t n;
+    const void* in[kMaxPay];
+    void* out[kMaxPay];
+    int32_t bytes[kMaxPay];
+    int32_t rowid[kMaxPay];
+};
+
+__device__ __forceinline__ uint32_t digit_of(const RadixKeys& k, uint64_t raw, int shift, uint32_t mask)
+{
+    uint64_t v;
+    if (k.mode == DIG_HASH) v = (k.bytes <= 4) ? (uint64_t)dev::fmix32((uint32_t)raw) : dev::fmix64(raw);
+    else v = twiddle_bits(raw, k.type, k.bytes) - k.kmin;
+    const uint32_t d = (uint32_t)(v >> shift) & mask;
+    return k.desc ? mask - d : d;
+}
+
+__global__ __launch_bounds__(kRB) void radix_hist_kernel(RadixKeys k, int64_t n, int shift, uint32_t mask,
+                                                          uint32_t* __restrict__ hist, int ntiles)
+{
+    __shared__ uint32_t h[256];
+    const int t = threadIdx.x;
+    h[t] = 0u;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRT;
+#pragma unroll
+    for (int i = 0; i < kRI; ++i) {
+        const int64_t e = base + (int64_t)i * kRB + t;
+        if (e < n) atomicAdd(&h[digit_of(k, dev::load_raw(k.data, k.bytes, e), shift, mask)], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
+}
+
+// exclusive scan of 256 values held one per thread (256 threads)
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wsum)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int i = 0; i < w; ++i) add += wsum[i];
+    __syncthreads();
+    return x - v + add;
+}
+
+__global__ __launch_bounds__(kRB) void radix_scatter_kernel(RadixKeys k, void* __restrict__ kout, PayArgs pay,
+                                                             int64_t n, int shift, int bits,
+                                                             const uint32_t* __restrict__ offs, int ntiles)
+{
+    __shared__ unsigned long long stage[kRT];
+    __shared__ uint8_t sdig[kRT];
+    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t woff[4][256];
+    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t lstart[256];
+    __shared__ uint32_t gofs[256];
+    __shared__ uint32_t wsum[4];
+
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const uint32_t mask = (1u << bits) - 1u;
+    const int64_t tbase = (int64_t)blockIdx.x * kRT;
+    cnt[t] = 0u;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) wcnt[w][t] = 0u;
+    gofs[t] = offs[(int64_t)t * ntiles + blockIdx.x];
+    __syncthreads();
+
+    uint32_t dg[kRI], pos[kRI];
+    const uint64_t lt = dev::lanemask_lt();
+#pragma unroll
+    for (int i = 0; i < kRI; ++i) {
+        const int64_t e = tbase + (int64_t)i * kRB + t;
+        const bool valid = e < n;
+        dg[i] = valid ? digit_of(k, dev::load_raw(k.data, k.bytes, e), shift, mask) : 0u;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < bits; ++b) {
+            const bool bit = (dg[i] >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0u) wcnt[wave][dg[i]] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t run = cnt[t];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const uint32_t c = wcnt[w][t];
+                woff[w][t] = run;
+                run += c;
+                wcnt[w][t] = 0u;
+            }
+            cnt[t] = run;
+        }
+        __syncthreads();
+        pos[i] = valid ? woff[wave][dg[i]] + rank : 0xffffffffu;
+    }
+    const uint32_t ls = block_excl_scan256(cnt[t], wsum);
+    lstart[t] = ls;
+    __syncthreads();
+    const int32_t tcount = (int32_t)min<int64_t>((int64_t)kRT, n - tbase);
+#pragma unroll
+    for (int i = 0; i < kRI; ++i)
+        if (pos[i] != 0xffffffffu) {
+            pos[i] += lstart[dg[i]];
+            sdig[pos[i]] = (uint8_t)dg[i];
+        }
+
+    // keys, then every payload column, through the LDS stage
+    for (int c = -1; c < pay.n; ++c) {
+        const void* src = c < 0 ? k.data : pay.in[c];
+        void* dst = c < 0 ? kout : pay.out[c];
+        const int bytes = c < 0 ? k.bytes : pay.bytes[c];
+        const bool rid = c >= 0 && pay.rowid[c] != 0;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kRI; ++i)
+            if (pos[i] != 0xffffffffu) {
+                const int64_t e = tbase + (int64_t)i * kRB + t;
+                stage[pos[i]] = rid ? (unsigned long long)e : (unsigned long long)dev::load_raw(src, bytes, e);
+            }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kRI; ++i) {
+            const int p = i * kRB + t;
+            if (p < tcount) {
+                const uint32_t d = sdig[p];
+                dev::store_bytes(dst, bytes, (int64_t)gofs[d] + (p - (int64_t)lstart[d]), stage[p]);
+            }
+        }
+    }
+}
+
+// ---- device-wide exclusive scan of uint32 (reduce, scan the block sums, add) ----
+constexpr int kSB = 256, kSI = 16, kST = kSB * kSI;
+
+__global__ __launch_bounds__(kSB) void scan_reduce_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                          uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t wsum[4];
+    const int64_t base = (int64_t)blockIdx.x * kST;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < kSI; ++i) {
+        const int64_t e = base + (int64_t)i * kSB + threadIdx.x;
+        if (e < n) acc += in[e];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) sums[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Scans one kST chunk per block (blocked arrangement: thread t owns kSI
+// consecutive values), adding the chunk's offset from `chunk_offs`.
+__global__ __launch_bounds__(kSB) void scan_chunks_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                          int64_t n, const uint32_t* __restrict__ chunk_offs)
+{
+    __shared__ uint32_t wsum[4];
+    const int64_t base = (int64_t)blockIdx.x * kST + (int64_t)threadIdx.x * kSI;
+    uint32_t v[kSI];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int i = 0; i < kSI; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0u;
+        tot += v[i];
+    }
+    uint32_t ex = block_excl_scan256(tot, wsum) + (chunk_offs ? chunk_offs[blockIdx.x] : 0u);
+#pragma unroll
+    for (int i = 0; i < kSI; ++i) {
+        if (base + i < n) out[base + i] = ex;
+        ex += v[i];
+    }
+}
+
+__global__ void read_last_kernel(const uint32_t* in, const uint32_t* ex, int64_t n, unsigned long long* total)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = (unsigned long long)ex[n - 1] + in[n - 1];
+}
+
+void scan_rec(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s)
+{
+    const int64_t chunks = (n + kST - 1) / kST;
+    if (chunks == 1) {
+        hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(kSB), 0, s, in, out, n, (const uint32_t*)nullptr);
+        EG_LAUNCH_CHECK();
+        return;
+    }
+    DevBuf sums((size_t)chunks * 4, s), offs((size_t)chunks * 4, s);
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)chunks), dim3(kSB), 0, s, in, n, sums.get<uint32_t>());
+    EG_LAUNCH_CHECK();
+    scan_rec(sums.get<uint32_t>(), offs.get<uint32_t>(), chunks, s);
+    hipLaunchKernelGGL(scan_chunks_kernel, dim3((unsigned)chunks), dim3(kSB), 0, s, in, out, n,
+                       (const uint32_t*)offs.get<uint32_t>());
+    EG_LAUNCH_CHECK();
+}
+
+// ---- key range ----------------------------------------------------------------------
+__global__ __launch_bounds__(256) void key_range_kernel(const void* keys, int32_t type, int bytes, int64_t n,
+                                                        unsigned long long* mn, unsigned long long* mx)
+{
+    uint64_t lo = ~0ull, hi = 0ull;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t v = twiddle_bits(dev::load_raw(keys, bytes, i), type, bytes);
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint64_t a = __shfl_xor((unsigned long long)lo, off), b = __shfl_xor((unsigned long long)hi, off);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(mn, (unsigned long long)lo);
+        atomicMax(mx, (unsigned long long)hi);
+    }
+}
+
+__global__ void init_range_kernel(unsigned long long* p)
+{
+    if (threadIdx.x == 0) {
+        p[0] = ~0ull;
+        p[1] = 0ull;
+    }
+}
+
+}  // namespace
+
+uint64_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, bool want_total)
+{
+    if (n <= 0) return 0;
+    scan_rec(in, out, n, s);
+    if (!want_total) return 0;
+    DevBuf tot(8, s);
+    hipLaunchKernelGGL(read_last_kernel, dim3(1), dim3(64), 0, s, in, (const uint32_t*)out, n,
+                       tot.get<unsigned long long>());
+    EG_LAUNCH_CHECK();
+    int64_t* h = pinned_i64(1);
+    EG_HIP(hipMemcpyAsync(h, tot.get(), 8, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    return (uint64_t)h[0];
+}
+
+void key_range(const void* keys, int32_t type, int bytes, int64_t n, hipStream_t s, uint64_t* mn, uint64_t* mx)
+{
+    DevBuf r(16, s);
+    hipLaunchKernelGGL(init_range_kernel, dim3(1), dim3(64), 0, s, r.get<unsigned long long>());
+    EG_LAUNCH_CHECK();
+    if (n > 0) {
+        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)device_info().num_cus * 8));
+        KernelTimer timer("key_range", s);
+        hipLaunchKernelGGL(key_range_kernel, dim3((unsigned)grid), dim3(256), 0, s, keys, type, bytes, n,
+                           r.get<unsigned long long>(), r.get<unsigned long long>() + 1);
+        EG_LAUNCH_CHECK();
+    }
+    int64_t* h = pinned_i64(2);
+    EG_HIP(hipMemcpyAsync(h, r.get(), 16, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    *mn = (uint64_t)h[0];
+    *mx = (uint64_t)h[1];
+}
+
+void radix_pass(const RadixKeys& k, void* keys_out, const std::vector<Payload>& pay, int64_t n, int shift,
+                int bits, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (n >= (int64_t)1 << 32) raise(EG_ERR_UNSUPPORTED, "radix pass: at most 2^32 - 1 rows");
+    if (bits < 1 || bits > 8) raise(EG_ERR_INVALID, "radix pass: 1..8 digit bits");
+    if ((int)pay.size() > kMaxPay) raise(EG_ERR_UNSUPPORTED, "radix pass: too many payload columns");
+    const int ntiles = (int)((n + kRT - 1) / kRT);
+    const uint32_t mask = (1u << bits) - 1u;
+    DevBuf hist((size_t)256 * ntiles * 4, s), offs((size_t)256 * ntiles * 4, s);
+    {
+        KernelTimer timer("radix_hist", s);
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(ntiles), dim3(kRB), 0, s, k, n, shift, mask, hist.get<uint32_t>(),
+                           ntiles);
+        EG_LAUNCH_CHECK();
+    }
+    scan_exclusive_u32(hist.get<uint32_t>(), offs.get<uint32_t>(), (int64_t)256 * ntiles, s, false);
+    PayArgs pa{};
+    pa.n = (int32_t)pay.size();
+    for (size_t c = 0; c < pay.size(); ++c) {
+        pa.in[c] = pay[c].in;
+        pa.out[c] = pay[c].out;
+        pa.bytes[c] = pay[c].bytes;
+        pa.rowid[c] = pay[c].rowid;
+    }
+    KernelTimer timer("radix_scatter", s);
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3(ntiles), dim3(kRB), 0, s, k, keys_out, pa, n, shift, bits,
+                       (const uint32_t*)offs.get<uint32_t>(), ntiles);
+    EG_LAUNCH_CHECK();
+}
+
+void radix_sort(const RadixKeys& k0, SortBuffers& b, int64_t n, int begin_bit, int end_bit, hipStream_t s,
+                const void** keys_res, std::vector<const void*>& pay_res)
+{
+    const size_t np = b.pay_in.size();
+    pay_res.assign(np, nullptr);
+    if (end_bit <= begin_bit || n <= 0) {
+        // no digit bits: the input order is already sorted; copy it to A
+        if (n > 0) {
+            EG_HIP(hipMemcpyAsync(b.keys_a, b.keys_in, (size_t)n * k0.bytes, hipMemcpyDeviceToDevice, s));
+            for (size_t c = 0; c < np; ++c) {
+                if (b.pay_in[c].rowid) {
+                    // generated row ids: one pass of zero digit bits is not possible, write iota
+                    std::vector<int64_t> iota((size_t)n);
+                    for (int64_t i = 0; i < n; ++i) iota[(size_t)i] = i;
+                    EG_HIP(hipMemcpyAsync(b.pay_a[c], iota.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+                    EG_HIP(hipStreamSynchronize(s));
+                } else {
+                    EG_HIP(hipMemcpyAsync(b.pay_a[c], b.pay_in[c].in, (size_t)n * b.pay_in[c].bytes,
+                                          hipMemcpyDeviceToDevice, s));
+                }
+            }
+        }
+        *keys_res = b.keys_a;
+        for (size_t c = 0; c < np; ++c) pay_res[c] = b.pay_a[c];
+        return;
+    }
+    RadixKeys k = k0;
+    const void* kin = b.keys_in;
+    std::vector<Payload> pay(np);
+    for (size_t c = 0; c < np; ++c) pay[c] = b.pay_in[c];
+    bool to_a = true;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        const int bits = std::min(8, end_bit - shift);
+        void* kout = to_a ? b.keys_a : b.keys_b;
+        for (size_t c = 0; c < np; ++c) pay[c].out = to_a ? b.pay_a[c] : b.pay_b[c];
+        k.data = kin;
+        radix_pass(k, kout, pay, n, shift, bits, s);
+        kin = kout;
+        for (size_t c = 0; c < np; ++c) {
+            pay[c].in = pay[c].out;
+            if (pay[c].rowid) {
+                pay[c].rowid = 0;
+                pay[c].bytes = 8;
+            }
+        }
+        to_a = !to_a;
+    }
+    *keys_res = kin;
+    for (size_t c = 0; c < np; ++c) pay_res[c] = pay[c].in;
+}
+
+}  // namespace eg

@@ -91,6 +91,21 @@ const DeviceInfo& device_info()
     return cache.emplace(dev, d).first->second;
 }
 
+size_t enable_dynamic_lds(const void* kernel)
+{
+    static std::mutex mu;
+    static std::unordered_map<const void*, size_t> done;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = done.find(kernel);
+    if (it != done.end()) return it->second;
+    hipFuncAttributes fa{};
+    EG_HIP(hipFuncGetAttributes(&fa, kernel));
+    const size_t avail = (size_t)device_info().max_lds - fa.sharedSizeBytes;
+    EG_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail));
+    done.emplace(kernel, avail);
+    return avail;
+}
+
 // ---- memory manager (mirrors gamax::MemoryManager, memory/mem_manager.hpp:12-158) --
 namespace {
 struct Alloc {

@@ -841,7 +841,9 @@ int orc_aggregate(const orc_col* cols, int32_t ncols, const orc_req* reqs,
 
 /* Radix-sortable bits of element i: the key with its sign bit flipped for
  * signed integers, and the usual order-preserving transform for floats (the
- * ordering cub::DeviceRadixSort sorts by, operators/sort/sort.cu:15-66). */
+ * ordering cub::DeviceRadixSort sorts by, operators/sort/sort.cu:15-66).
+ * cub documents that -0.0 and +0.0 sort as equal keys: -0.0 takes the bits
+ * of +0.0 before the transform, so a stable sort keeps zeros in input order. */
 static uint64_t twiddle(const void* k, int32_t t, int64_t i)
 {
     uint64_t b = raw_bits(k, t, i);
@@ -850,6 +852,7 @@ static uint64_t twiddle(const void* k, int32_t t, int64_t i)
     if (is_signed_int(t)) return b ^ sign;
     if (is_float(t)) {
         uint64_t all = (w == 64) ? ~(uint64_t)0 : ((((uint64_t)1) << w) - 1);
+        if (b == sign) b = 0;
         return (b & sign) ? (~b & all) : (b ^ sign);
     }
     return b;

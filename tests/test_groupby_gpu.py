@@ -20,7 +20,12 @@ A = G.aggregation_id
 STRATEGIES = {
     "auto": G.groupby,
     "hash": G.hash_groupby,
+    "sort": G.sort_groupby,
+    "sort_gfur": G.sort_groupby_gfur,
+    "partition": G.partition_groupby,
+    "partition_gfur": G.partition_groupby_gfur,
 }
+ORDERED = {"sort", "sort_gfur"}  # strategies whose output is in ascending key order
 
 
 def run(fn, keys, vals, reqs, hint=None):
@@ -191,14 +196,38 @@ def test_groups_beyond_the_lds_table_spill_to_hbm(name, fn):
     assert_same(got, O.groupby(keys, [vals], reqs))
 
 
-def test_hash_table_overflow_retries_with_a_bigger_table():
-    """A groups_upper_bound far below the truth fills the HBM table; the
-    operator retries instead of returning a partial result."""
+@pytest.mark.parametrize("name", ["hash", "partition", "partition_gfur"])
+def test_group_bound_far_too_small_retries(name):
+    """A groups_upper_bound far below the truth fills the HBM table / output;
+    the operator retries instead of returning a partial result."""
     n = 1 << 18
     keys = np.arange(n, dtype=np.int32)[::-1].copy()
     vals = np.ones(n, dtype=np.int64)
     reqs = [(-1, A.COUNT_ALL), (0, A.SUM)]
-    got = sorted_by_key(run(G.hash_groupby, keys, [vals], reqs, G.groupby_hint(groups_upper_bound=10)))
+    got = sorted_by_key(run(STRATEGIES[name], keys, [vals], reqs, G.groupby_hint(groups_upper_bound=10)))
+    assert_same(got, O.groupby(keys, [vals], reqs))
+
+
+@pytest.mark.parametrize("name", ["sort", "sort_gfur"])
+def test_sort_strategy_output_is_key_ordered(name):
+    rng = np.random.default_rng(8)
+    keys = rng.integers(-(1 << 40), 1 << 40, 50_000).astype(np.int64)
+    keys = keys[rng.integers(0, 50_000, 200_000)]
+    vals = rng.integers(0, 100, 200_000).astype(np.int32)
+    out = run(STRATEGIES[name], keys, [vals], [(0, A.SUM)])
+    assert np.all(np.diff(out[0]) > 0)
+    assert_same(out, O.groupby(keys, [vals], [(0, A.SUM)]))
+
+
+@pytest.mark.parametrize("name", ["partition", "partition_gfur"])
+def test_partition_with_one_huge_partition_is_sliced(name):
+    """Zipf s=1.5: the hottest key holds ~40% of the rows; its partition is
+    split into slices merged through the HBM table."""
+    n = 1 << 22
+    keys = O.gen_keys_zipf(n, O.zipf_cdf(1 << 16, 1.5), seed=3)
+    vals = O.gen_values(n, O.VAL_UNIFORM_I64, np.int64, seed=3, stream_id=2)
+    reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MIN), (0, A.MAX)]
+    got = sorted_by_key(run(STRATEGIES[name], keys, [vals], reqs, G.groupby_hint(groups_upper_bound=1 << 16)))
     assert_same(got, O.groupby(keys, [vals], reqs))
 
 

@@ -308,6 +308,23 @@ def test_sort_pairs_is_a_stable_sort_on_the_bit_range(kt, bits, desc):
     np.testing.assert_array_equal(sk, keys[order])
 
 
+@pytest.mark.parametrize("kt", [np.float32, np.float64])
+@pytest.mark.parametrize("desc", [False, True])
+def test_sort_pairs_orders_floats_like_numpy_with_equal_zeros(kt, desc):
+    """Full-width float sort equals numpy's stable sort by value: negative
+    numbers before positive, infinities at the ends, and -0.0 / +0.0 equal
+    (cub::DeviceRadixSort's documented treatment of the two zeros)."""
+    rng = np.random.default_rng(4)
+    keys = (rng.standard_normal(5000) * 100).astype(kt)
+    keys[:12] = [0.0, -0.0, np.inf, -np.inf, 0.0, -0.0, 1e-30, -1e-30, 7.0, 7.0, -0.0, 0.0]
+    rng.shuffle(keys)
+    vals = np.arange(len(keys), dtype=np.int64)
+    sk, sv = O.sort_pairs(keys, vals, 0, -1, desc)
+    order = np.argsort(-keys if desc else keys, kind="stable")
+    np.testing.assert_array_equal(sv, order)
+    np.testing.assert_array_equal(sk.view(np.uint8), keys[order].view(np.uint8))
+
+
 def test_partition_offsets_are_inclusive_prefix_of_counts():
     keys = np.array([3, 1, 0, 3, 3, 2, 1], dtype=np.int32)
     off = O.partition_offsets(keys, 0, 2)
