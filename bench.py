@@ -13,7 +13,10 @@ One step = one group-by of every case.  value = rows / second over the whole
 sweep (Grows/s = N / t / 1e9, the reference's Elem/s).  Algorithmic bytes per
 case = 16 N + 40 G (each input column read once, each output written once).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--strategy hash]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--strategy auto]
+    --strategy auto   the library's dispatcher (HyperLogLog estimate + rule)
+    --strategy hash|sort|sort_gfur|partition|partition_gfur   one strategy
+    --explore         also time every strategy once per case (stderr + sweep)
 """
 import argparse
 import json
@@ -29,6 +32,13 @@ METRIC = "group-by Grows/s @256M rows, %HBM peak; learned-vs-fixed dispatch spee
 DS = [1 << 4, 1 << 10, 1 << 16, 1 << 22, 1 << 26]
 DISTS = [("uniform", 0.0), ("zipf", 0.5), ("zipf", 1.0), ("zipf", 1.5)]
 SEED = 42
+STRATS = {"auto": "groupby", "hash": "hash_groupby", "sort": "sort_groupby", "sort_gfur": "sort_groupby_gfur",
+          "partition": "partition_groupby", "partition_gfur": "partition_groupby_gfur"}
+# Algorithmic HBM bytes per row of one launch of each timed kernel (int32 key,
+# int64 + float32 values): inputs read once; a radix scatter reads and writes
+# the key and both values once (2 x 16 B).
+KERNEL_BYTES_PER_ROW = {"hash_agg": 16, "partition_agg": 16, "segreduce": 16, "radix_scatter": 32,
+                        "radix_hist": 4, "seg_heads": 4, "key_range": 4, "hll": 4, "partition_bounds": 4}
 
 
 def log(*a):
@@ -41,7 +51,8 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--rows", type=int, default=1 << 28)
-    p.add_argument("--strategy", default="hash", choices=["auto", "hash", "sort", "partition"])
+    p.add_argument("--strategy", default="auto", choices=list(STRATS))
+    p.add_argument("--explore", action="store_true")
     p.add_argument("--cases", default="all", help="'all' or comma list like 16:uniform,1024:zipf1.0")
     p.add_argument("--cpu-sample-rows", type=int, default=1 << 22)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -69,7 +80,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+              "(the operator runs on one GPU)", file=sys.stderr)
         sys.exit(2)
 
     from eiger_amd import gamax as G
@@ -77,8 +88,7 @@ def main():
     A = G.aggregation_id
     n = args.rows
     cases = case_list(args.cases)
-    fn = {"auto": G.groupby, "hash": G.hash_groupby, "sort": G.sort_groupby,
-          "partition": G.partition_groupby}[args.strategy]
+    fn = getattr(G, STRATS[args.strategy])
 
     # ---- inputs resident in HBM -------------------------------------------------
     t0 = time.time()
@@ -94,9 +104,9 @@ def main():
     log(f"generated {len(cases)} cases x {n} rows in {time.time() - t0:.1f}s")
     reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MIN), (0, A.MAX), (1, A.SUM)]
 
-    def run_case(i):
+    def run_case(i, f=fn):
         h = G.groupby_hint(groups_upper_bound=cases[i][0])
-        t = fn([keys[i]], [v64, vf], reqs, h)
+        t = f([keys[i]], [v64, vf], reqs, h)
         g = t.num_rows()
         t.free()
         return g
@@ -111,6 +121,16 @@ def main():
     for w in range(args.warmup):
         for i in range(len(cases)):
             groups[i] = run_case(i)
+    explore = {}
+    if args.explore:
+        for name in ("hash", "sort", "sort_gfur", "partition", "partition_gfur"):
+            f = getattr(G, STRATS[name])
+            for i in range(len(cases)):
+                run_case(i, f)  # the operator returns after its stream is synchronised
+                a = time.perf_counter()
+                run_case(i, f)
+                explore.setdefault(i, {})[name] = round((time.perf_counter() - a) * 1e3, 3)
+            log(f"explored {name}")
     G.kernel_timing_reset()
     G.kernel_timing(True)
     per_case = [0.0] * len(cases)
@@ -133,13 +153,15 @@ def main():
         b = 16 * n + 40 * groups[i]
         sweep.append({"case": case_name(c), "groups": groups[i], "ms": round(t * 1e3, 3),
                       "grows_per_s": round(n / t / 1e9, 2), "hbm_frac": round(b / t / PEAK_HBM, 4)})
+        if i in explore:
+            sweep[-1]["ms_by_strategy"] = explore[i]
         log(f"  {case_name(c):24s} G={groups[i]:>9d}  {t * 1e3:8.3f} ms  {n / t / 1e9:7.2f} Grows/s  "
-            f"{b / t / PEAK_HBM * 100:5.1f}% HBM")
+            f"{b / t / PEAK_HBM * 100:5.1f}% HBM" + (f"  {explore[i]}" if i in explore else ""))
 
     # ---- dominant kernel roofline --------------------------------------------------
     kernels = {}
-    for name in ("hash_agg", "hash_compact", "hll", "sort_pass", "partition_pass", "partition_agg",
-                 "segreduce"):
+    for name in ("hash_agg", "hash_compact", "hll", "key_range", "radix_hist", "radix_scatter", "seg_heads",
+                 "segreduce", "partition_bounds", "partition_agg"):
         cnt, ms = G.kernel_time(name)
         if cnt:
             kernels[name] = (cnt, ms)
@@ -148,11 +170,11 @@ def main():
         dom = max(kernels, key=lambda k: kernels[k][1])
         cnt, ms = kernels[dom]
         avg_s = ms / cnt * 1e-3
-        per_launch = 16 * n  # algorithmic bytes per launch: 16 B/row x rows (inputs read once)
+        per_launch = KERNEL_BYTES_PER_ROW.get(dom, 16) * n  # every launch covers the n rows of one case
         achieved = per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9,
                     "unit": "GB/s", "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": None,
-                    "launches": cnt, "avg_ms": round(ms / cnt, 4),
+                    "launches": cnt, "avg_ms": round(ms / cnt, 4), "bytes_per_row": KERNEL_BYTES_PER_ROW.get(dom, 16),
                     "bytes_per_launch": per_launch, "copy_probe_gbs": round(copy_gbs, 1)}
         log(f"dominant kernel {dom}: {cnt} launches, avg {ms / cnt:.3f} ms, {achieved:.0f} GB/s algorithmic")
         for k, (c, m) in kernels.items():

@@ -33,6 +33,37 @@ class Hint(C.Structure):
                 ("allow_hash_fallback", C.c_int32)]
 
 
+BATCH_MAX_AGGS = 4
+
+
+class BatchQuery(C.Structure):
+    """eg_batch_query: one query of a batched group-by."""
+    _fields_ = [("keys", C.c_void_p), ("vals", C.c_void_p), ("rows", C.c_int64),
+                ("group_bound", C.c_int64), ("capacity", C.c_int64), ("out_keys", C.c_void_p),
+                ("out_aggs", C.c_void_p * BATCH_MAX_AGGS)]
+
+
+class BatchTuning(C.Structure):
+    """eg_batch_tuning: execution parameters of a batched group-by."""
+    _fields_ = [("tile_rows_log2", C.c_int32), ("lds_slots_log2", C.c_int32),
+                ("lds_max_bound_log2", C.c_int32), ("blocks_per_cu", C.c_int32)]
+
+
+MLP_PARAMS = 5508
+
+
+class MlpQuery(C.Structure):
+    """eg_mlp_query: one key column to featurise."""
+    _fields_ = [("keys", C.c_void_p), ("rows", C.c_int64), ("group_bound", C.c_int64), ("key_type", C.c_int32),
+                ("nops", C.c_int32), ("value_bytes", C.c_int32), ("reserved", C.c_int32)]
+
+
+class MlpTrainConfig(C.Structure):
+    """eg_mlp_train_config."""
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("lam", C.c_float),
+                ("weight_decay", C.c_float), ("seed", C.c_uint64)]
+
+
 class LibraryNotBuilt(RuntimeError):
     pass
 
@@ -88,6 +119,24 @@ def _proto(L):
     f("eg_zipf_cdf", C.c_int, [vp, i64, C.c_double])
     f("eg_gen_keys_zipf", C.c_int, [vp, i32, i64, i64, vp, i64, u64, vp])
     f("eg_gen_values", C.c_int, [vp, i32, i32, i64, i64, u64, u32, vp])
+    PMQ, PF = C.POINTER(MlpQuery), C.POINTER(C.c_float)
+    f("eg_mlp_features", C.c_int, [PMQ, i64, vp, vp])
+    f("eg_mlp_forward", C.c_int, [vp, vp, i64, vp, vp])
+    f("eg_mlp_loss_grad", C.c_int, [vp, vp, vp, vp, i64, C.c_float, vp, PF, vp])
+    f("eg_mlp_train", C.c_int, [vp, vp, vp, vp, i64, C.POINTER(MlpTrainConfig), PF, vp])
+    f("eg_mlp_load", C.c_int, [vp, i64])
+    f("eg_mlp_loaded", C.c_int, [])
+    f("eg_mlp_select", C.c_int, [ColumnView, i64, i32, i32, vp, C.POINTER(i32), C.POINTER(i64)])
+    PBQ, PBT = C.POINTER(BatchQuery), C.POINTER(BatchTuning)
+    f("eg_batch_tuning_default", None, [PBT])
+    f("eg_batch_plan_create", C.c_int, [PBQ, i64, i32, i32, PAR, i64, PBT, vp, PT])
+    f("eg_batch_plan_tuning", C.c_int, [vp, PBT])
+    f("eg_batch_run", C.c_int, [vp, vp])
+    f("eg_batch_graph_launch", C.c_int, [vp, vp])
+    f("eg_batch_results", C.c_int, [vp, vp, vp, vp])
+    f("eg_batch_plan_free", None, [vp])
+    f("eg_batch_autotune", C.c_int, [PBQ, i64, i32, i32, PAR, i64, PBT, i64, i32, vp, PBT,
+                                     C.POINTER(C.c_double)])
 
 
 def lib():

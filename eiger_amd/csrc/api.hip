@@ -24,6 +24,8 @@ void timing_query(const char* name, int64_t* launches, double* total_ms);
 void zipf_cdf(double* cdf, int64_t distinct, double s);
 Result run_sort(const Plan& plan, const eg_groupby_hint& hint, bool gfur, hipStream_t s);
 Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream_t s);
+bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_bound, int nops, int value_bytes,
+                hipStream_t s, int32_t* algo, int64_t* card_est);
 int64_t ndv_approx(const void* data, int32_t type, int64_t n, hipStream_t s);
 void scalar_aggregate(const std::vector<ViewIn>& cols, const eg_agg_request* reqs, int64_t nreqs,
                       hipStream_t s, Result& res);
@@ -38,7 +40,7 @@ struct PlanBuilder {
     {
         auto it = vcol_of.find(idx);
         if (it != vcol_of.end()) return it->second;
-        if (p.spec.nvcols >= kMaxVCols) raise(EG_ERR_UNSUPPORTED, "too many distinct value columns");
+        if (p.spec.nvcols >= kMaxVCols) throw WideError("too many distinct value columns");
         int c = p.spec.nvcols++;
         p.spec.cols[c].data = v.data;
         p.spec.cols[c].type = v.type;
@@ -53,7 +55,7 @@ struct PlanBuilder {
     {
         for (int a = 0; a < p.spec.nops; ++a)
             if (p.spec.ops[a].code == code && p.spec.ops[a].col == col) return a;
-        if (p.spec.nops >= kMaxOps) raise(EG_ERR_UNSUPPORTED, "too many aggregation states");
+        if (p.spec.nops >= kMaxOps) throw WideError("too many aggregation states");
         int a = p.spec.nops++;
         p.spec.ops[a].code = code;
         p.spec.ops[a].col = col;
@@ -258,6 +260,51 @@ int64_t group_bound_of(const Plan& p, const eg_groupby_hint& h, hipStream_t s)
 
 enum class Entry { TOP, HASH, SORT, SORT_GFUR, PARTITION, PARTITION_GFUR };
 
+// Requests that need more states or value columns than one pass holds: the
+// requests are split into chunks that fit, each chunk is aggregated by the
+// sort strategy (rows in ascending key order, so every chunk's rows line up),
+// and the result is [key, every request's column in order, trailing count].
+Result wide_groupby(const ViewIn& k, const std::vector<ViewIn>& vs, const eg_agg_request* reqs, int64_t nreqs,
+                    bool top, bool agg_count, const eg_groupby_hint& h, hipStream_t s)
+{
+    Result all;
+    int64_t groups = -1;
+    int64_t r0 = 0;
+    while (r0 < nreqs) {
+        // the longest prefix of the remaining requests that fits one plan
+        int64_t take = 0;
+        Plan plan;
+        for (int64_t m = 1; r0 + m <= nreqs; ++m) {
+            const bool last = r0 + m == nreqs;
+            try {
+                plan = make_plan(&k, vs, reqs + r0, m, top && agg_count && last, top);
+                take = m;
+            } catch (const WideError&) {
+                break;
+            }
+        }
+        if (take == 0) raise(EG_ERR_UNSUPPORTED, "one aggregation request needs more states than a pass holds");
+        plan = make_plan(&k, vs, reqs + r0, take, top && agg_count && r0 + take == nreqs, top);
+        Result part;
+        if (k.size == 0) {
+            OutSpec os;
+            alloc_outputs(plan, 0, s, part, os);
+        } else {
+            part = run_sort(plan, h, false, s);
+        }
+        if (groups >= 0 && part.num_groups != groups) raise(EG_ERR_DEVICE, "chunked group-by: group counts differ");
+        groups = part.num_groups;
+        for (size_t j = 0; j < part.cols.size(); ++j) {
+            if (j == 0 && !all.cols.empty()) continue;  // the key column once
+            all.cols.push_back(std::move(part.cols[j]));
+            all.types.push_back(part.types[j]);
+        }
+        r0 += take;
+    }
+    all.num_groups = groups;
+    return all;
+}
+
 int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
                   const eg_column_view* vals, int64_t num_vals, const eg_agg_request* reqs,
                   int64_t num_reqs, eg_groupby_hint* hint, void* stream, eg_table* out)
@@ -277,9 +324,18 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
         std::vector<ViewIn> vs;
         for (int64_t i = 0; i < num_vals; ++i) vs.push_back(view_in(vals[i]));
         const bool top = entry == Entry::TOP;
-        Plan plan = make_plan(&k, vs, reqs, num_reqs, top && h.agg_count != 0, top);
+        Plan plan;
+        try {
+            plan = make_plan(&k, vs, reqs, num_reqs, top && h.agg_count != 0, top);
+        } catch (const WideError&) {
+            Result res = wide_groupby(k, vs, reqs, num_reqs, top, h.agg_count != 0, h, s);
+            EG_HIP(hipStreamSynchronize(s));
+            *out = to_table(res);
+            return;
+        }
 
         int algo;
+        int64_t learned_bound = 0;  // group bound from the dispatch model's estimate
         switch (entry) {
         case Entry::HASH: algo = EG_ALGO_HASH; break;
         case Entry::SORT: algo = EG_ALGO_SORT; break;
@@ -288,8 +344,20 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
         case Entry::PARTITION_GFUR: algo = EG_ALGO_PARTITION_GFUR; break;
         default:
             if (h.groupby_algo == EG_ALGO_AUTO && plan.rows > 0) {
-                int rc = eg_auto_select_algo(keys, num_keys, &h, stream);
-                if (rc != EG_OK) raise(rc, last_error_cstr());
+                // learned dispatch when a model is loaded, else the reference's rule
+                int vbytes = 0;
+                for (int c = 0; c < plan.spec.nvcols; ++c) vbytes += plan.spec.cols[c].bytes;
+                int32_t a = 0;
+                int64_t card = 0;
+                if (mlp_select(plan.key.data, plan.key_type, plan.rows, h.groups_upper_bound, plan.spec.nops, vbytes, s,
+                               &a, &card)) {
+                    h.groupby_algo = a;
+                    if (!(h.groups_upper_bound > 0 && h.groups_upper_bound < plan.rows))
+                        learned_bound = std::min<int64_t>(plan.rows, card + card + card / 2 + 1024);
+                } else {
+                    int rc = eg_auto_select_algo(keys, num_keys, &h, stream);
+                    if (rc != EG_OK) raise(rc, last_error_cstr());
+                }
             }
             algo = h.groupby_algo;
             if (algo == EG_ALGO_AUTO) algo = EG_ALGO_HASH;  // empty input
@@ -303,10 +371,12 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
             alloc_outputs(plan, 0, s, res, os);
         } else {
             switch (algo) {
-            case EG_ALGO_HASH: res = run_hash(plan, group_bound_of(plan, h, s), s); break;
+            case EG_ALGO_HASH: res = run_hash(plan, learned_bound > 0 ? learned_bound : group_bound_of(plan, h, s), s); break;
             case EG_ALGO_SORT: case EG_ALGO_SORT_OPT: res = run_sort(plan, h, false, s); break;
             case EG_ALGO_SORT_GFUR: res = run_sort(plan, h, true, s); break;
-            case EG_ALGO_PARTITION: res = run_partition(plan, group_bound_of(plan, h, s), false, s); break;
+            case EG_ALGO_PARTITION:
+                res = run_partition(plan, learned_bound > 0 ? learned_bound : group_bound_of(plan, h, s), false, s);
+                break;
             case EG_ALGO_PARTITION_GFUR: res = run_partition(plan, group_bound_of(plan, h, s), true, s); break;
             default: raise(EG_ERR_UNSUPPORTED, "Unsupported groupby algorithm");
             }

@@ -21,9 +21,15 @@ struct GTable {
     int W;
 };
 
+// Longest probe sequence before the table counts as full.  Tables are sized
+// for a load of at most 1/2, where linear probing stays far below this; a
+// table that reaches it is (nearly) full and is rebuilt larger.
+constexpr uint64_t kMaxProbes = 2048;
+
 // Slot of encoded key `enc` (inserted if absent); nullptr when the table is
 // full (meta[2] is then set and the caller retries with a larger table).
-__device__ __forceinline__ unsigned long long* gt_slot(const GTable& gt, uint64_t enc)
+// `ins` counts this lane's insertions; gt_count_inserts publishes the count.
+__device__ __forceinline__ unsigned long long* gt_slot(const GTable& gt, uint64_t enc, uint32_t& ins)
 {
     if (enc == 0) {
         if (__hip_atomic_load(&gt.meta[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
@@ -31,14 +37,15 @@ __device__ __forceinline__ unsigned long long* gt_slot(const GTable& gt, uint64_
         return gt.side;
     }
     uint64_t h = dev::fmix64(enc) & gt.mask;
-    for (uint64_t p = 0; p <= gt.mask; ++p) {
+    const uint64_t limit = gt.mask < kMaxProbes ? gt.mask + 1 : kMaxProbes;
+    for (uint64_t p = 0; p < limit; ++p) {
         unsigned long long* slot = gt.base + h * (uint64_t)gt.W;
         uint64_t cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur == enc) return slot;
         if (cur == 0) {
             uint64_t prev = atomicCAS(slot, 0ull, (unsigned long long)enc);
             if (prev == 0) {
-                atomicAdd(&gt.meta[0], 1ull);
+                ++ins;
                 return slot;
             }
             if (prev == enc) return slot;
@@ -47,6 +54,15 @@ __device__ __forceinline__ unsigned long long* gt_slot(const GTable& gt, uint64_
     }
     atomicExch(&gt.meta[2], 1ull);
     return nullptr;
+}
+
+// Adds the wave's insertion count to meta[0] with one atomic.  Every lane of
+// the wave calls it (at the end of a kernel).
+__device__ __forceinline__ void gt_count_inserts(const GTable& gt, uint32_t ins)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) ins += __shfl_xor(ins, off);
+    if (dev::lane_id() == 0 && ins != 0u) atomicAdd(&gt.meta[0], (unsigned long long)ins);
 }
 
 int table_words_per_slot(int nops);

@@ -24,6 +24,12 @@ struct Error : std::runtime_error {
 
 [[noreturn]] inline void raise(int status, const std::string& msg) { throw Error(status, msg); }
 
+// A request list with more aggregation states or value columns than one
+// kernel pass holds; the top level splits it (api.hip).
+struct WideError : Error {
+    explicit WideError(const std::string& m) : Error(EG_ERR_UNSUPPORTED, m) {}
+};
+
 #define EG_HIP(call)                                                                         \
     do {                                                                                     \
         hipError_t eg_e_ = (call);                                                           \

@@ -207,19 +207,19 @@ __device__ __forceinline__ bool lds_row(LdsTable<KB>& tab, bool use_lds, int r_m
 // One row into the HBM table (called by every lane of the wave).
 template <int KB, int NV>
 __device__ __forceinline__ void hbm_row(const GTable& gt, const KeyDesc& key, const AggSpec& spec, bool pending,
-                                        uint64_t kraw, const uint64_t (&cv)[NV], int peel_min)
+                                        uint64_t kraw, const uint64_t (&cv)[NV], int peel_min, uint32_t& ins)
 {
     const uint64_t e64 = dev::enc64(dev::widen_key(kraw, KB, key.is_signed));
     bool done = false;
     if (peel_min <= 64)
         done = dev::wave_peel(pending, e64, spec, peel_min, [&](int a) { return operand_of<NV>(spec, a, cv); },
                               [&](uint64_t t) {
-                                  unsigned long long* g = gt_slot(gt, t);
+                                  unsigned long long* g = gt_slot(gt, t, ins);
                                   return g ? g + 1 : (unsigned long long*)nullptr;
                               },
                               (int64_t)1);
     if (pending && !done) {
-        unsigned long long* g = gt_slot(gt, e64);
+        unsigned long long* g = gt_slot(gt, e64, ins);
         if (g) dev::apply_ops<NV>(spec, cv, [&](int a) { return g + 1 + a; });
     }
 }

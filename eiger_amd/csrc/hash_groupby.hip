@@ -43,6 +43,59 @@ using namespace lds;
 template <int NV>
 constexpr int max_block() { return NV <= 2 ? 1024 : 512; }
 
+// Merges the replicas of the block's LDS table and adds every group to the
+// HBM table.
+template <int KB>
+__device__ __forceinline__ void flush_lds(LdsTable<KB>& tab, const GTable& gt, const KeyDesc& key, const AggSpec& spec,
+                          int rep_log2, int r_mask, uint32_t& ins)
+{
+    using KT = typename LdsKey<KB>::T;
+    __syncthreads();
+    // merge replicas 1..R-1 into replica 0 (LDS atomics)
+    const int R = 1 << rep_log2;
+    if (R > 1) {
+        for (int idx = threadIdx.x; idx < tab.T; idx += blockDim.x) {
+            const int r = idx & r_mask;
+            if (r == 0) continue;
+            const int s = idx >> rep_log2;
+            int dst;
+            KT kw = 0;
+            if (s == tab.cr) {
+                if (tab.side[r] == 0ull) continue;
+                dst = tab.find((KT)0, 0, 1);
+            } else {
+                kw = tab.keys[idx];
+                if (kw == 0) continue;
+                dst = tab.find(kw, 0, tab.cr);
+            }
+            auto src = [&](int a) { return tab.st[(size_t)a * tab.T + idx]; };
+            if (dst >= 0) {
+                dev::merge_states(spec, [&](int a) { return &tab.st[(size_t)a * tab.T + dst]; }, src);
+            } else {
+                // replica 0 is full: merge this slot straight into HBM
+                unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed), ins);
+                if (g) dev::merge_states(spec, [&](int a) { return g + 1 + a; }, src);
+            }
+        }
+        __syncthreads();
+    }
+    // flush replica 0 into the HBM table
+    for (int s = threadIdx.x; s <= tab.cr; s += blockDim.x) {
+        const int idx = s << rep_log2;
+        KT kw;
+        if (s == tab.cr) {
+            if (tab.side[0] == 0ull) continue;
+            kw = 0;
+        } else {
+            kw = tab.keys[idx];
+            if (kw == 0) continue;
+        }
+        unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed), ins);
+        if (!g) continue;
+        dev::merge_states(spec, [&](int a) { return g + 1 + a; }, [&](int a) { return tab.st[(size_t)a * tab.T + idx]; });
+    }
+}
+
 template <int KB, class VL>
 __global__ __launch_bounds__(max_block<VL::NV>()) void hash_agg_kernel(KeyDesc key, int64_t n, AggSpec spec,
                                                                         GTable gt, int cr_log2, int rep_log2,
@@ -61,6 +114,7 @@ __global__ __launch_bounds__(max_block<VL::NV>()) void hash_agg_kernel(KeyDesc k
         __syncthreads();
     }
     const int r_mask = (1 << (use_lds ? rep_log2 : 0)) - 1;
+    uint32_t ins = 0;  // HBM-table insertions by this lane
 
     // ---- full tiles: vector loads, next tile in flight while this one runs ----
     const int64_t tile = (int64_t)blockDim.x * kRows;
@@ -91,7 +145,7 @@ __global__ __launch_bounds__(max_block<VL::NV>()) void hash_agg_kernel(KeyDesc k
 #pragma unroll
                 for (int c = 0; c < NV; ++c) raw[c] = pick4(vcur[c], i);
                 canon_row<VL, NV>(spec, raw, cv);
-                hbm_row<KB, NV>(gt, key, spec, pend, pick4(kcur, i), cv, peel_min);
+                hbm_row<KB, NV>(gt, key, spec, pend, pick4(kcur, i), cv, peel_min, ins);
             }
         }
 #pragma unroll
@@ -112,54 +166,10 @@ __global__ __launch_bounds__(max_block<VL::NV>()) void hash_agg_kernel(KeyDesc k
             raw[c] = (valid && c < spec.nvcols) ? dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, r) : 0ull;
         canon_row<VL, NV>(spec, raw, cv);
         const bool sp = lds_row<KB, NV>(tab, use_lds, r_mask, lane, key, spec, kraw, cv, valid, probe_limit, peel_min);
-        if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kraw, cv, peel_min);
+        if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kraw, cv, peel_min, ins);
     }
-
-    if (!use_lds) return;
-    __syncthreads();
-    // merge replicas 1..R-1 into replica 0 (LDS atomics)
-    const int R = 1 << rep_log2;
-    if (R > 1) {
-        for (int idx = threadIdx.x; idx < tab.T; idx += blockDim.x) {
-            const int r = idx & r_mask;
-            if (r == 0) continue;
-            const int s = idx >> rep_log2;
-            int dst;
-            KT kw = 0;
-            if (s == tab.cr) {
-                if (tab.side[r] == 0ull) continue;
-                dst = tab.find((KT)0, 0, 1);
-            } else {
-                kw = tab.keys[idx];
-                if (kw == 0) continue;
-                dst = tab.find(kw, 0, tab.cr);
-            }
-            auto src = [&](int a) { return tab.st[(size_t)a * tab.T + idx]; };
-            if (dst >= 0) {
-                dev::merge_states(spec, [&](int a) { return &tab.st[(size_t)a * tab.T + dst]; }, src);
-            } else {
-                // replica 0 is full: merge this slot straight into HBM
-                unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed));
-                if (g) dev::merge_states(spec, [&](int a) { return g + 1 + a; }, src);
-            }
-        }
-        __syncthreads();
-    }
-    // flush replica 0 into the HBM table
-    for (int s = threadIdx.x; s <= tab.cr; s += blockDim.x) {
-        const int idx = s << rep_log2;
-        KT kw;
-        if (s == tab.cr) {
-            if (tab.side[0] == 0ull) continue;
-            kw = 0;
-        } else {
-            kw = tab.keys[idx];
-            if (kw == 0) continue;
-        }
-        unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed));
-        if (!g) continue;
-        dev::merge_states(spec, [&](int a) { return g + 1 + a; }, [&](int a) { return tab.st[(size_t)a * tab.T + idx]; });
-    }
+    if (use_lds) flush_lds(tab, gt, key, spec, rep_log2, r_mask, ins);
+    gt_count_inserts(gt, ins);
 }
 
 __global__ void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter)

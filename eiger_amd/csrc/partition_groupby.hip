@@ -81,6 +81,7 @@ __global__ __launch_bounds__(kPB) void part_agg_kernel(KeyDesc key, AggSpec spec
     __syncthreads();
 
     const Unit u = units[blockIdx.x];
+    uint32_t ins = 0;  // HBM-table insertions by this lane
     for (int64_t b = u.start; b < u.end; b += blockDim.x) {
         const int64_t r = b + threadIdx.x;
         const bool valid = r < u.end;
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(kPB) void part_agg_kernel(KeyDesc key, AggSpec spec
 #pragma unroll
         for (int c = 0; c < NV; ++c) cv[c] = c < spec.nvcols ? dev::canon(raw[c], spec.cols[c].conv) : 0ull;
         const bool sp = lds_row<KB, NV>(tab, true, 0, lane, key, spec, kraw, cv, valid, probe_limit, peel_min);
-        if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kraw, cv, peel_min);
+        if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kraw, cv, peel_min, ins);
     }
     __syncthreads();
     // finished groups of this unit
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(kPB) void part_agg_kernel(KeyDesc key, AggSpec spec
         const uint64_t enc = lds_to_enc64<KB>(kw, key.is_signed);
         if (u.sliced) {
             if (occ) {
-                unsigned long long* g = gt_slot(gt, enc);
+                unsigned long long* g = gt_slot(gt, enc, ins);
                 if (g)
                     dev::merge_states(spec, [&](int a) { return g + 1 + a; },
                                       [&](int a) { return tab.st[(size_t)a * tab.T + s]; });
@@ -123,6 +124,7 @@ __global__ __launch_bounds__(kPB) void part_agg_kernel(KeyDesc key, AggSpec spec
             }
         }
     }
+    gt_count_inserts(gt, ins);
 }
 
 // Appends the HBM table's groups after the directly written ones.
@@ -292,8 +294,12 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     EG_HIP(hipMemcpyAsync(dunits.get(), units.data(), units.size() * sizeof(Unit), hipMemcpyHostToDevice, s));
 
     // ---- aggregation ------------------------------------------------------------------
+    // HBM table: groups of sliced partitions plus LDS spills
+    int64_t sliced_parts = 0;
+    for (int64_t p = 0; p < P; ++p) sliced_parts += (ho[(size_t)p + 1] - ho[(size_t)p]) > slice ? 1 : 0;
     int64_t capacity = bound;
     int64_t slots = 1 << 14;
+    while (slots < 4 * std::min<int64_t>(bound, sliced_parts * per_part * 2)) slots <<= 1;
     const int W = table_words_per_slot(nops);
     for (;;) {
         Result res;

@@ -1,0 +1,215 @@
+"""Trace generation, training and evaluation of the learned strategy dispatcher.
+
+    python -m eiger_amd.dispatch_train --train 50000 --heldout 10000 --out DIR
+
+1. Queries are drawn like the two benchmark workloads: (a) batched uneven
+   queries -- rows log-uniform in [1e3, 4e6], distinct count log-uniform in
+   [1, rows/2], uniform or Zipf s ~ U[0.5, 1.5], SUM(int64) + COUNT; (b)
+   strategy-sweep queries -- rows log-uniform in [2^20, 2^23], distinct count
+   log-uniform in [2^4, 2^26] (capped by rows), uniform or Zipf s in {0.5, 1,
+   1.5}, COUNT + SUM/MIN/MAX(int64) + SUM(float32).  Keys are int32, generated
+   on the device.
+2. Each query's 16 sample features come from the HIP feature kernel; each
+   strategy (hash, sort, partition) is timed on it, with the true group count
+   as the hint (the label is what each strategy costs when sized right).
+3. The MLP is trained on the device by eg_mlp_train (HIP forward/backward +
+   Adam) against soft targets p_i ~ exp(-(t_i / t_best - 1) / tau) and the
+   log2 group count.
+4. Held-out evaluation: total time of the strategies each dispatcher picks,
+   plus each dispatcher's own cost (the rule's HyperLogLog pass over the whole
+   key column; the MLP's sample features + inference), against the
+   exhaustive best per query and against fixed single strategies.
+
+Writes DIR/dispatch_mlp.f32 (5508 little-endian float32), DIR/dispatch_eval.json
+and DIR/dispatch_trace.npz.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import gamax as G
+
+A = G.aggregation_id
+T = G.type_id
+STRATS = ("hash", "sort", "partition")
+FNS = {"hash": G.hash_groupby, "sort": G.sort_groupby, "partition": G.partition_groupby}
+ALGO_INDEX = {G.groupby_algo.HASH: 0, G.groupby_algo.SORT: 1, G.groupby_algo.PARTITION: 2}
+REQS_BATCH = [(0, A.SUM), (-1, A.COUNT_ALL)]
+REQS_SWEEP = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MIN), (0, A.MAX), (1, A.SUM)]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def draw_queries(n: int, seed: int):
+    """Query specs: (kind, rows, distinct, zipf_s)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        if rng.random() < 0.5:
+            rows = int(math.exp(rng.uniform(math.log(1e3), math.log(4e6))))
+            d = int(math.exp(rng.uniform(0.0, math.log(max(rows / 2, 1.0)))))
+            s = 0.0 if rng.random() < 0.5 else float(rng.uniform(0.5, 1.5))
+            out.append(("batch", rows, max(d, 1), s))
+        else:
+            rows = int(2 ** rng.uniform(20, 23))
+            d = int(2 ** rng.uniform(4, 26))
+            d = max(1, min(d, rows))
+            s = float(rng.choice([0.0, 0.5, 1.0, 1.5]))
+            out.append(("sweep", rows, d, s))
+    return out
+
+
+class Workload:
+    """Device-resident value columns shared by every query (prefix views)."""
+
+    def __init__(self, max_rows: int):
+        self.v64 = G.gen_values(max_rows, G.val_dist.UNIFORM_I64, T.INT64, seed=42, stream_id=2)
+        self.vf = G.gen_values(max_rows, G.val_dist.UNIFORM_F32, T.FLOAT32, seed=42, stream_id=3)
+
+    def query(self, spec, seed: int):
+        kind, rows, d, s = spec
+        if s == 0.0:
+            k = G.gen_keys_uniform(rows, d, seed=seed, t=T.INT32)
+        else:
+            k = G.gen_keys_zipf(rows, d, s, seed=seed, t=T.INT32)
+        if kind == "batch":
+            vals = [G.column_view(self.v64.data, rows, T.INT64)]
+            reqs, nops, vb = REQS_BATCH, 2, 8
+        else:
+            vals = [G.column_view(self.v64.data, rows, T.INT64), G.column_view(self.vf.data, rows, T.FLOAT32)]
+            reqs, nops, vb = REQS_SWEEP, 5, 12
+        return k, vals, reqs, nops, vb
+
+
+def timed(fn, *args):
+    a = time.perf_counter()
+    r = fn(*args)
+    return r, (time.perf_counter() - a) * 1e3
+
+
+def trace(specs, seed0: int, wl: Workload, progress_every: int = 500):
+    """Features, per-strategy times (ms), group counts, rule choice and the
+    two dispatchers' own costs for every query."""
+    n = len(specs)
+    feats = np.zeros((n, 16), np.float32)
+    times = np.zeros((n, 3), np.float64)
+    groups = np.zeros(n, np.int64)
+    rule = np.zeros(n, np.int32)
+    rule_ms = np.zeros(n, np.float64)
+    feat_ms = np.zeros(n, np.float64)
+    t0 = time.time()
+    for i, spec in enumerate(specs):
+        k, vals, reqs, nops, vb = wl.query(spec, seed0 + i)
+        f, feat_ms[i] = timed(G.mlp_features, [k], None, nops, vb)
+        feats[i] = f[0]
+        h = G.groupby_hint()
+        _, rule_ms[i] = timed(G.auto_select_groupby_algo, [k], h)
+        rule[i] = ALGO_INDEX.get(h.groupby_algo, 0)
+        for j, name in enumerate(STRATS):
+            hint = G.groupby_hint(groups_upper_bound=spec[2])
+            t, times[i, j] = timed(FNS[name], [k], vals, reqs, hint)
+            if j == 0:
+                groups[i] = t.num_rows()
+            t.free()
+        k.free()
+        if (i + 1) % progress_every == 0:
+            log(f"traced {i + 1}/{n} queries in {time.time() - t0:.0f}s")
+    return dict(features=feats, times=times, groups=groups, rule=rule, rule_ms=rule_ms, feat_ms=feat_ms)
+
+
+def soft_targets(times: np.ndarray, tau: float) -> np.ndarray:
+    rel = times / times.min(axis=1, keepdims=True) - 1.0
+    w = np.exp(-rel / tau)
+    return (w / w.sum(axis=1, keepdims=True)).astype(np.float32)
+
+
+def evaluate(params: np.ndarray, tr: dict, infer_ms: float) -> dict:
+    x, t = tr["features"], tr["times"]
+    n = len(x)
+    y = G.mlp_forward(params, x)
+    pick = np.argmax(y[:, :3], axis=1)
+    best = t.min(axis=1)
+    mlp_total = float(t[np.arange(n), pick].sum() + tr["feat_ms"].sum() + infer_ms * n)
+    rule_total = float(t[np.arange(n), tr["rule"]].sum() + tr["rule_ms"].sum())
+    out = {
+        "queries": n,
+        "best_ms": float(best.sum()),
+        "mlp_ms": mlp_total,
+        "rule_ms": rule_total,
+        "mlp_choice_only_ms": float(t[np.arange(n), pick].sum()),
+        "rule_choice_only_ms": float(t[np.arange(n), tr["rule"]].sum()),
+        "mlp_overhead_ms": float(tr["feat_ms"].sum() + infer_ms * n),
+        "rule_overhead_ms": float(tr["rule_ms"].sum()),
+        "fixed_ms": {s: float(t[:, j].sum()) for j, s in enumerate(STRATS)},
+        "mlp_over_best": mlp_total / float(best.sum()),
+        "rule_over_mlp": rule_total / mlp_total,
+        "mlp_picks_best": float((pick == t.argmin(axis=1)).mean()),
+        "rule_picks_best": float((tr["rule"] == t.argmin(axis=1)).mean()),
+        "mlp_within_10pct": float((t[np.arange(n), pick] <= 1.1 * best).mean()),
+        "card_log2_mae": float(np.abs(y[:, 3] * 32 - np.log2(np.maximum(tr["groups"], 1))).mean()),
+    }
+    return out
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--train", type=int, default=50000)
+    p.add_argument("--heldout", type=int, default=10000)
+    p.add_argument("--epochs", type=int, default=80)
+    p.add_argument("--tau", type=float, default=0.05)
+    p.add_argument("--out", default="dispatch_out")
+    args = p.parse_args()
+    os.makedirs(args.out, exist_ok=True)
+
+    specs_tr = draw_queries(args.train, seed=1)
+    specs_ho = draw_queries(args.heldout, seed=2)
+    wl = Workload(max(s[1] for s in specs_tr + specs_ho))
+    # warm every strategy once
+    for spec in specs_tr[:8]:
+        k, vals, reqs, _, _ = wl.query(spec, 0)
+        for name in STRATS:
+            FNS[name]([k], vals, reqs, G.groupby_hint(groups_upper_bound=spec[2])).free()
+        k.free()
+    log("tracing training queries")
+    tr = trace(specs_tr, 1_000_000, wl)
+    log("tracing held-out queries")
+    ho = trace(specs_ho, 2_000_000, wl)
+
+    target = soft_targets(tr["times"], args.tau)
+    card = (np.log2(np.maximum(tr["groups"], 1)) / 32).astype(np.float32)
+    P0 = G.mlp_init(0)
+    a = time.perf_counter()
+    params, losses = G.mlp_train(P0, tr["features"], target, card, epochs=args.epochs, batch=1024, lr=2e-3,
+                                 lam=4.0, weight_decay=1e-5, seed=3)
+    train_s = time.perf_counter() - a
+    log(f"trained {args.epochs} epochs in {train_s:.1f}s: loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+
+    # inference cost per query: features are timed in the trace; forward of one sample here
+    G.mlp_forward(params, ho["features"][:1])
+    a = time.perf_counter()
+    for i in range(50):
+        G.mlp_forward(params, ho["features"][i:i + 1])
+    infer_ms = (time.perf_counter() - a) * 1e3 / 50
+
+    ev = {"train": evaluate(params, tr, infer_ms), "heldout": evaluate(params, ho, infer_ms),
+          "epochs": args.epochs, "tau": args.tau, "train_seconds": train_s,
+          "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]), "infer_ms": infer_ms}
+    log(json.dumps(ev["heldout"], indent=1))
+    params.astype("<f4").tofile(os.path.join(args.out, "dispatch_mlp.f32"))
+    with open(os.path.join(args.out, "dispatch_eval.json"), "w") as f:
+        json.dump(ev, f, indent=1)
+    np.savez_compressed(os.path.join(args.out, "dispatch_trace.npz"),
+                        **{f"train_{k}": v for k, v in tr.items()}, **{f"heldout_{k}": v for k, v in ho.items()})
+
+
+if __name__ == "__main__":
+    main()

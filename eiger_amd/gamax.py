@@ -20,6 +20,7 @@ from typing import Iterable, List, Optional, Sequence
 
 import numpy as np
 
+from . import _lib as _lib_mod
 from ._lib import AggRequest, ColumnView, Hint, lib
 
 MAXIMUM_SIZE = (1 << 63) - 1
@@ -492,3 +493,192 @@ def gen_values(n: int, dist: int, t: int, seed: int = 42, stream_id: int = 2, be
     col = make_column(t, n, stream=stream)
     _check(lib().eg_gen_values(C.c_void_p(col.data), int(t), int(dist), begin, n, seed, stream_id, _stream(stream)))
     return col
+
+
+# ---------------------------------------------------------------- batched group-by
+class batch_query:
+    """One query of a batched group-by: key column, optional value column,
+    group upper bound, and caller-owned output columns of `capacity` rows."""
+
+    def __init__(self, keys: column_view, vals: Optional[column_view], group_bound: int,
+                 out_keys: column_view, out_aggs: Sequence[column_view], capacity: Optional[int] = None):
+        self.keys, self.vals, self.group_bound = keys, vals, int(group_bound)
+        self.out_keys, self.out_aggs = out_keys, list(out_aggs)
+        self.capacity = int(out_keys.size if capacity is None else capacity)
+
+
+def batch_tuning(tile_rows_log2=None, lds_slots_log2=None, lds_max_bound_log2=None, blocks_per_cu=None):
+    t = _lib_mod.BatchTuning()
+    lib().eg_batch_tuning_default(C.byref(t))
+    for k, v in (("tile_rows_log2", tile_rows_log2), ("lds_slots_log2", lds_slots_log2),
+                 ("lds_max_bound_log2", lds_max_bound_log2), ("blocks_per_cu", blocks_per_cu)):
+        if v is not None:
+            setattr(t, k, int(v))
+    return t
+
+
+def _batch_array(queries: Sequence[batch_query]):
+    arr = (_lib_mod.BatchQuery * max(len(queries), 1))()
+    for i, q in enumerate(queries):
+        a = arr[i]
+        a.keys = q.keys.data + q.keys.offset * size_of(q.keys.type) if q.keys.data else None
+        a.vals = (q.vals.data + q.vals.offset * size_of(q.vals.type)) if (q.vals is not None and q.vals.data) else None
+        a.rows = q.keys.size
+        a.group_bound = q.group_bound
+        a.capacity = q.capacity
+        a.out_keys = q.out_keys.data or None
+        for j, c in enumerate(q.out_aggs):
+            a.out_aggs[j] = c.data or None
+    return arr
+
+
+class batch_plan:
+    """A planned batch: run() / graph_launch() execute it asynchronously on
+    the stream; results() synchronises and returns (num_groups, status)."""
+
+    def __init__(self, queries: Sequence[batch_query], key_type: int, val_type: int, requests, tuning=None,
+                 stream=None):
+        self._nq = len(queries)
+        self._arr = _batch_array(queries)
+        rq = _reqs(requests)
+        h = C.c_void_p()
+        _check(lib().eg_batch_plan_create(self._arr, self._nq, int(key_type), int(val_type), rq, len(requests),
+                                          C.byref(tuning) if tuning is not None else None, _stream(stream),
+                                          C.byref(h)))
+        self._h = h
+
+    def tuning(self):
+        t = _lib_mod.BatchTuning()
+        _check(lib().eg_batch_plan_tuning(self._h, C.byref(t)))
+        return t
+
+    def run(self, stream=None):
+        _check(lib().eg_batch_run(self._h, _stream(stream)))
+
+    def graph_launch(self, stream=None):
+        _check(lib().eg_batch_graph_launch(self._h, _stream(stream)))
+
+    def results(self, stream=None):
+        ng = np.zeros(self._nq, dtype=np.int64)
+        st = np.zeros(self._nq, dtype=np.int32)
+        _check(lib().eg_batch_results(self._h, ng.ctypes.data_as(C.c_void_p), st.ctypes.data_as(C.c_void_p),
+                                      _stream(stream)))
+        return ng, st
+
+    def free(self):
+        if self._h:
+            lib().eg_batch_plan_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def batch_autotune(queries: Sequence[batch_query], key_type: int, val_type: int, requests, candidates,
+                   reps: int = 2, stream=None):
+    """Times every candidate tuning on this batch; returns (best, times_ms)."""
+    arr = _batch_array(queries)
+    cand = (_lib_mod.BatchTuning * len(candidates))(*candidates)
+    times = np.zeros(len(candidates), dtype=np.float64)
+    best = _lib_mod.BatchTuning()
+    _check(lib().eg_batch_autotune(arr, len(queries), int(key_type), int(val_type), _reqs(requests), len(requests),
+                                   cand, len(candidates), int(reps), _stream(stream), C.byref(best),
+                                   times.ctypes.data_as(C.POINTER(C.c_double))))
+    return best, times
+
+
+# ---------------------------------------------------------------- learned dispatch
+MLP_PARAMS = _lib_mod.MLP_PARAMS
+MLP_FEATURES = 16
+
+
+def _dev_f32(a) -> column:
+    return make_column_from_host(np.ascontiguousarray(a, dtype=np.float32).ravel())
+
+
+def mlp_features(keys: Sequence[column_view], group_bounds=None, nops=2, value_bytes=8, stream=None) -> np.ndarray:
+    """The 16 dispatch features of each key column (computed on the device)."""
+    n = len(keys)
+    arr = (_lib_mod.MlpQuery * max(n, 1))()
+    for i, k in enumerate(keys):
+        arr[i].keys = (k.data + k.offset * size_of(k.type)) if k.data else None
+        arr[i].rows = k.size
+        arr[i].group_bound = int(group_bounds[i]) if group_bounds is not None else 0
+        arr[i].key_type = k.type
+        arr[i].nops = int(nops[i]) if hasattr(nops, "__len__") else int(nops)
+        arr[i].value_bytes = int(value_bytes[i]) if hasattr(value_bytes, "__len__") else int(value_bytes)
+    out = make_column(type_id.FLOAT32, max(n, 1) * MLP_FEATURES, stream=stream)
+    _check(lib().eg_mlp_features(arr, n, C.c_void_p(out.data), _stream(stream)))
+    f = out.to_numpy()[: n * MLP_FEATURES].reshape(n, MLP_FEATURES)
+    out.free()
+    return f
+
+
+def mlp_forward(params: np.ndarray, x: np.ndarray, stream=None) -> np.ndarray:
+    P, X = _dev_f32(params), _dev_f32(x)
+    n = x.shape[0]
+    Y = make_column(type_id.FLOAT32, max(n, 1) * 4, stream=stream)
+    _check(lib().eg_mlp_forward(C.c_void_p(P.data), C.c_void_p(X.data), n, C.c_void_p(Y.data), _stream(stream)))
+    y = Y.to_numpy()[: n * 4].reshape(n, 4)
+    for c in (P, X, Y):
+        c.free()
+    return y
+
+
+def mlp_loss_grad(params, x, target, card, lam, stream=None):
+    P, X, T, Cc = _dev_f32(params), _dev_f32(x), _dev_f32(target), _dev_f32(card)
+    Gd = make_column(type_id.FLOAT32, MLP_PARAMS, stream=stream)
+    loss = C.c_float()
+    _check(lib().eg_mlp_loss_grad(C.c_void_p(P.data), C.c_void_p(X.data), C.c_void_p(T.data), C.c_void_p(Cc.data),
+                                  x.shape[0], float(lam), C.c_void_p(Gd.data), C.byref(loss), _stream(stream)))
+    g = Gd.to_numpy()
+    for c in (P, X, T, Cc, Gd):
+        c.free()
+    return loss.value, g
+
+
+def mlp_train(params, x, target, card, epochs=60, batch=1024, lr=2e-3, lam=1.0, weight_decay=0.0, seed=1,
+              stream=None):
+    """Adam training on the device; returns (params, loss per epoch)."""
+    P, X, T, Cc = _dev_f32(params), _dev_f32(x), _dev_f32(target), _dev_f32(card)
+    cfg = _lib_mod.MlpTrainConfig(int(epochs), int(batch), float(lr), float(lam), float(weight_decay), int(seed))
+    losses = np.zeros(int(epochs), dtype=np.float32)
+    _check(lib().eg_mlp_train(C.c_void_p(P.data), C.c_void_p(X.data), C.c_void_p(T.data), C.c_void_p(Cc.data),
+                              x.shape[0], C.byref(cfg), losses.ctypes.data_as(C.POINTER(C.c_float)), _stream(stream)))
+    out = P.to_numpy().astype(np.float32)
+    for c in (P, X, T, Cc):
+        c.free()
+    return out, losses
+
+
+def mlp_init(seed=0) -> np.ndarray:
+    """He-initialised parameters (host, float32)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(MLP_PARAMS, dtype=np.float32)
+    p[:1024] = rng.standard_normal(1024) * np.sqrt(2 / 16)
+    p[1088:5184] = rng.standard_normal(4096) * np.sqrt(2 / 64)
+    p[5248:5504] = rng.standard_normal(256) * np.sqrt(1 / 64)
+    return p
+
+
+def mlp_load(params: Optional[np.ndarray]):
+    """Loads a dispatch model (None unloads); eg_groupby AUTO then uses it."""
+    if params is None:
+        _check(lib().eg_mlp_load(None, 0))
+        return
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    _check(lib().eg_mlp_load(p.ctypes.data_as(C.c_void_p), p.size))
+
+
+def mlp_loaded() -> bool:
+    return bool(lib().eg_mlp_loaded())
+
+
+def mlp_select(key: column_view, group_bound=0, nops=2, value_bytes=8, stream=None):
+    a, card = C.c_int32(), C.c_int64()
+    _check(lib().eg_mlp_select(key._to_c(), int(group_bound), int(nops), int(value_bytes), _stream(stream),
+                               C.byref(a), C.byref(card)))
+    return groupby_algo(a.value), card.value

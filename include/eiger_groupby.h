@@ -18,7 +18,8 @@
  *   - Column data are device pointers.  Inputs are never modified or freed.
  *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
  *     Single-query entry points are synchronous at return, as the reference's
- *     (groupby.cu:390).  eg_batch_groupby is asynchronous and graph-capturable.
+ *     (groupby.cu:390).  eg_batch_run / eg_batch_graph_launch are asynchronous
+ *     and graph-capturable.
  *   - Result tables (eg_table) own their device columns; free them with
  *     eg_table_free, or take a column with eg_table_release_column and free its
  *     data with eg_free.
@@ -229,6 +230,112 @@ int eg_compute_ndv_approx(eg_column_view col, void* stream, int64_t* out_ndv);
  * hint->groupby_algo in place when it is EG_ALGO_AUTO. */
 int eg_auto_select_algo(const eg_column_view* keys, int64_t num_keys,
                         eg_groupby_hint* hint, void* stream);
+
+/* ---- batched group-by (no reference counterpart) -----------------------------
+ * Many independent single-key group-bys executed by one persistent launch.
+ * The reference runs one group-by per call, each synchronous
+ * (operators/groupby/groupby.hpp:44-49, groupby.cu:390); this entry point is
+ * for callers that hold thousands of small and uneven queries.  Every query
+ * has one key column (key_type) and at most one value column (val_type,
+ * column_index 0 in every request); the requests (SUM/MAX/MIN/ANY/COUNT_ALL)
+ * are the same for all queries.  Outputs are caller-allocated columns of
+ * `capacity` rows: [out_keys, out_aggs[j] per request], typed as
+ * eg_agg_output_type.  Row order is unspecified.
+ *
+ * eg_batch_plan_create allocates the plan's device tables and uploads the
+ * work list (synchronous).  eg_batch_run and eg_batch_graph_launch are
+ * asynchronous, make no allocation and no host synchronisation, so the
+ * execution can be captured in a hipGraph (eg_batch_graph_launch captures it
+ * once on its first call and replays the graph afterwards).  Queries whose
+ * group_bound is below their true group count, or whose groups exceed
+ * `capacity`, come back with a non-zero status from eg_batch_results. */
+#define EG_BATCH_MAX_AGGS 4
+enum { EG_BATCH_TABLE_FULL = 1, EG_BATCH_CAPACITY = 2 };
+typedef struct eg_batch_query {
+    const void* keys;        /* device, rows elements of key_type */
+    const void* vals;        /* device, rows elements of val_type (NULL if only COUNT_ALL) */
+    int64_t rows;
+    int64_t group_bound;     /* upper bound of the distinct keys (<= 0: rows) */
+    int64_t capacity;        /* rows available in each output column */
+    void* out_keys;
+    void* out_aggs[EG_BATCH_MAX_AGGS];
+} eg_batch_query;
+/* Execution parameters picked by eg_batch_autotune. */
+typedef struct eg_batch_tuning {
+    int32_t tile_rows_log2;      /* rows per work item (10..26) */
+    int32_t lds_slots_log2;      /* LDS table slots per workgroup (6..13) */
+    int32_t lds_max_bound_log2;  /* queries with group_bound <= 2^this use the LDS table */
+    int32_t blocks_per_cu;       /* persistent workgroups per CU (1..8) */
+} eg_batch_tuning;
+typedef struct eg_batch_plan_s* eg_batch_plan;
+void eg_batch_tuning_default(eg_batch_tuning* t);
+int eg_batch_plan_create(const eg_batch_query* queries, int64_t num_queries, int32_t key_type,
+                         int32_t val_type, const eg_agg_request* reqs, int64_t num_reqs,
+                         const eg_batch_tuning* tuning, void* stream, eg_batch_plan* out);
+/* The tuning the plan runs with (LDS slots clamped to what fits). */
+int eg_batch_plan_tuning(eg_batch_plan plan, eg_batch_tuning* out);
+int eg_batch_run(eg_batch_plan plan, void* stream);
+int eg_batch_graph_launch(eg_batch_plan plan, void* stream);
+/* Synchronises `stream`; num_groups / status: num_queries entries (host). */
+int eg_batch_results(eg_batch_plan plan, int64_t* num_groups, int32_t* status, void* stream);
+void eg_batch_plan_free(eg_batch_plan plan);
+/* Times every candidate tuning on this batch (reps timed runs after a
+ * warm-up, hipEvents on `stream`) and returns the fastest; times_ms
+ * (optional, num_candidates entries) receives each candidate's mean time. */
+int eg_batch_autotune(const eg_batch_query* queries, int64_t num_queries, int32_t key_type,
+                      int32_t val_type, const eg_agg_request* reqs, int64_t num_reqs,
+                      const eg_batch_tuning* candidates, int64_t num_candidates, int32_t reps,
+                      void* stream, eg_batch_tuning* best, double* times_ms);
+
+/* ---- learned dispatch (no reference counterpart) ----------------------------
+ * A 16 -> 64 -> 64 -> 4 ReLU MLP (fp32) over 16 features of a 4096-row key
+ * sample.  Outputs 0..2: logits of {HASH, SORT, PARTITION}; output 3: predicted
+ * log2(group count) / 32.  Parameters: one flat array of EG_MLP_PARAMS floats,
+ * [W1 64x16 | b1 64 | W2 64x64 | b2 64 | W3 4x64 | b3 4], row-major (out, in).
+ * The rule of auto_select_groupby_algo (groupby.cu:68-119) stays the fallback:
+ * eg_groupby with EG_ALGO_AUTO uses the MLP only when a model is loaded
+ * (eg_mlp_load) and its outputs are finite.  The choice never changes a
+ * result, only the strategy that computes it. */
+#define EG_MLP_FEATURES 16
+#define EG_MLP_OUTPUTS 4
+#define EG_MLP_PARAMS 5508
+typedef struct eg_mlp_query {
+    const void* keys;        /* device */
+    int64_t rows;
+    int64_t group_bound;     /* hint (<= 0 or >= rows: unknown) */
+    int32_t key_type;
+    int32_t nops;            /* aggregation states of the query */
+    int32_t value_bytes;     /* bytes per row of the distinct value columns */
+    int32_t reserved;
+} eg_mlp_query;
+typedef struct eg_mlp_train_config {
+    int32_t epochs;
+    int32_t batch;           /* multiple of 64 */
+    float lr;                /* Adam step size */
+    float lambda;            /* weight of the log-cardinality loss */
+    float weight_decay;
+    uint64_t seed;           /* minibatch shuffling */
+} eg_mlp_train_config;
+/* out_features: num_queries x EG_MLP_FEATURES floats, device. */
+int eg_mlp_features(const eg_mlp_query* queries, int64_t num_queries, float* out_features, void* stream);
+/* params, x (n x 16), y (n x 4): device. */
+int eg_mlp_forward(const float* params, const float* x, int64_t n, float* y, void* stream);
+/* Mean loss over n samples and its gradient (EG_MLP_PARAMS floats, device):
+ * cross-entropy of softmax(y[0..2]) against target (n x 3, rows summing to 1)
+ * plus lambda * (y[3] - card)^2. */
+int eg_mlp_loss_grad(const float* params, const float* x, const float* target, const float* card, int64_t n,
+                     float lambda, float* grad, float* loss, void* stream);
+/* Adam training of params in place (device) over shuffled minibatches;
+ * loss_per_epoch (host, optional) receives each epoch's mean loss. */
+int eg_mlp_train(float* params, const float* x, const float* target, const float* card, int64_t n,
+                 const eg_mlp_train_config* cfg, float* loss_per_epoch, void* stream);
+/* Loads a model for the dispatcher (host array; NULL unloads). */
+int eg_mlp_load(const float* host_params, int64_t n);
+int eg_mlp_loaded(void);
+/* The loaded model's strategy (EG_ALGO_HASH/SORT/PARTITION) and group-count
+ * estimate for one key column. */
+int eg_mlp_select(eg_column_view key, int64_t group_bound, int32_t nops, int32_t value_bytes, void* stream,
+                  int32_t* algo, int64_t* card_est);
 
 /* ---- kernel timing (measurement support; no reference counterpart) -------- */
 int eg_kernel_timing_enable(int32_t on);

@@ -101,6 +101,12 @@ def lib():
         L.orc_rule_select.argtypes = [i32, i64]
         L.orc_count_distinct.restype = i64
         L.orc_count_distinct.argtypes = [vp, i32, i64]
+        L.orc_mlp_forward.restype = C.c_int
+        L.orc_mlp_forward.argtypes = [vp, vp, i64, vp]
+        L.orc_mlp_loss_grad.restype = C.c_int
+        L.orc_mlp_loss_grad.argtypes = [vp, vp, vp, vp, i64, C.c_double, vp, vp]
+        L.orc_sample_features.restype = C.c_int
+        L.orc_sample_features.argtypes = [vp, i32, i64, i64, i32, i32, vp]
         _lib = L
     return _lib
 
@@ -282,3 +288,39 @@ def rule_select(key_bytes: int, ndv: int) -> int:
 def count_distinct(keys) -> int:
     keys = np.ascontiguousarray(keys)
     return int(lib().orc_count_distinct(_ptr(keys), type_id(keys.dtype), len(keys)))
+
+
+# ---------------------------------------------------------------- learned dispatch
+MLP_PARAMS = 5508
+
+
+def mlp_forward(params, x):
+    """fp64 forward pass: params (5508,), x (n, 16) float32 -> (n, 4) float64."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty((x.shape[0], 4), dtype=np.float64)
+    lib().orc_mlp_forward(_ptr(p), _ptr(x), x.shape[0], _ptr(y))
+    return y
+
+
+def mlp_loss_grad(params, x, target, card, lam):
+    """fp64 mean loss and gradient (see eg_mlp_loss_grad)."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(target, dtype=np.float32)
+    c = np.ascontiguousarray(card, dtype=np.float32)
+    g = np.empty(MLP_PARAMS, dtype=np.float64)
+    loss = C.c_double()
+    lib().orc_mlp_loss_grad(_ptr(p), _ptr(x), _ptr(t), _ptr(c), x.shape[0], float(lam), _ptr(g), C.byref(loss))
+    return loss.value, g
+
+
+def sample_features(keys, group_bound=0, nops=2, value_bytes=8):
+    """The 16 dispatch features of a key column (float32)."""
+    keys = np.ascontiguousarray(keys)
+    out = np.empty(16, dtype=np.float32)
+    rc = lib().orc_sample_features(_ptr(keys), type_id(keys.dtype), len(keys), int(group_bound), int(nops),
+                                   int(value_bytes), _ptr(out))
+    if rc != 0:
+        raise OracleError("sample_features failed")
+    return out

@@ -117,6 +117,14 @@ int orc_hash_partition(const void* keys, int32_t key_type, int64_t n,
 int32_t orc_rule_select(int32_t key_bytes, int64_t ndv);
 int64_t orc_count_distinct(const void* keys, int32_t type, int64_t n);
 
+/* ---- learned dispatch (eg_oracle_mlp.c): fp64 restatement ---------------- */
+#define ORC_MLP_PARAMS 5508
+int orc_mlp_forward(const double* params, const float* x, int64_t n, double* y);
+int orc_mlp_loss_grad(const double* params, const float* x, const float* target, const float* card, int64_t n,
+                      double lambda, double* grad, double* loss);
+int orc_sample_features(const void* keys, int32_t key_type, int64_t rows, int64_t group_bound, int32_t nops,
+                        int32_t value_bytes, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -349,3 +349,18 @@ def test_full_size_sweep_shape_invariants():
     assert int(s.astype(np.int64).sum()) == int(tot[0][0])
     assert np.all(mn <= mx)
     assert abs(float(fs.astype(np.float64).sum()) - float(tot[1][0])) / float(tot[1][0]) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["auto", "hash", "partition"])
+def test_wide_request_lists_are_split_into_passes(name):
+    """More aggregation states (or value columns) than one pass holds: the
+    requests are aggregated in chunks whose key-ordered rows line up."""
+    rng = np.random.default_rng(12)
+    n = 100_000
+    keys = rng.integers(0, 700, n).astype(np.int32)
+    vals = [rng.integers(-1000, 1000, n).astype(np.int64) for _ in range(10)]
+    reqs = [(c, k) for c in range(10) for k in (A.SUM, A.MIN, A.MAX)] + [(-1, A.COUNT_ALL)]
+    got = sorted_by_key(run(STRATEGIES[name], keys, vals, reqs))
+    assert_same(got, O.groupby(keys, vals, reqs))
+    empty = run(STRATEGIES[name], keys[:0], [v[:0] for v in vals], reqs)
+    assert len(empty) == len(reqs) + 1 and all(len(c) == 0 for c in empty)

@@ -106,6 +106,48 @@ def test_fmix32_against_murmurhash3_vectors():
         assert _murmur3_x86_32(data, seed) == exp, (data, seed)
 
 
+def _murmur3_x64_128(data: bytes, seed: int = 0) -> bytes:
+    """MurmurHash3_x64_128 (public domain, A. Appleby) with the oracle's fmix64."""
+    M = (1 << 64) - 1
+    c1, c2 = 0x87c37b91114253d5, 0x4cf5ad432745937f
+
+    def rotl(x, r):
+        return ((x << r) | (x >> (64 - r))) & M
+
+    h1 = h2 = seed
+    n = len(data)
+    for i in range(n // 16):
+        k1 = int.from_bytes(data[16 * i:16 * i + 8], "little")
+        k2 = int.from_bytes(data[16 * i + 8:16 * i + 16], "little")
+        h1 ^= (rotl((k1 * c1) & M, 31) * c2) & M
+        h1 = (rotl(h1, 27) + h2) & M
+        h1 = (h1 * 5 + 0x52dce729) & M
+        h2 ^= (rotl((k2 * c2) & M, 33) * c1) & M
+        h2 = (rotl(h2, 31) + h1) & M
+        h2 = (h2 * 5 + 0x38495ab5) & M
+    tail = data[16 * (n // 16):]
+    if len(tail) > 8:
+        h2 ^= (rotl((int.from_bytes(tail[8:], "little") * c2) & M, 33) * c1) & M
+    if len(tail) > 0:
+        h1 ^= (rotl((int.from_bytes(tail[:8], "little") * c1) & M, 31) * c2) & M
+    h1 ^= n
+    h2 ^= n
+    h1 = (h1 + h2) & M
+    h2 = (h2 + h1) & M
+    h1, h2 = O.fmix64(h1), O.fmix64(h2)
+    h1 = (h1 + h2) & M
+    h2 = (h2 + h1) & M
+    return h1.to_bytes(8, "little") + h2.to_bytes(8, "little")
+
+
+def test_fmix64_against_murmurhash3_x64_128_vectors():
+    """Published MurmurHash3_x64_128 digests (seed 0) pin fmix64, the 8-byte
+    key finalizer (groupby_partition.cu:64-90)."""
+    assert _murmur3_x64_128(b"The quick brown fox jumps over the lazy dog").hex() == \
+        "6c1b07bc7bbc4be347939ac4a93c437a"
+    assert _murmur3_x64_128(b"").hex() == "00" * 16
+
+
 def test_fmix64_is_the_invertible_murmur_finalizer():
     c1, c2 = 0xff51afd7ed558ccd, 0xc4ceb9fe1a85ec53
     i1, i2 = pow(c1, -1, 1 << 64), pow(c2, -1, 1 << 64)
@@ -348,3 +390,68 @@ def test_rule_dispatcher_thresholds():
     assert O.rule_select(4, (1 << 14) + 1) == O.PARTITION
     assert O.rule_select(8, 1 << 18) == O.HASH
     assert O.rule_select(8, (1 << 18) + 1) == O.PARTITION
+
+
+# ------------------------------------------------------------------ learned dispatch oracle
+def _np_mlp(P, x):
+    """Textbook numpy forward pass of the 16-64-64-4 ReLU network."""
+    W1 = P[:1024].reshape(64, 16); b1 = P[1024:1088]
+    W2 = P[1088:5184].reshape(64, 64); b2 = P[5184:5248]
+    W3 = P[5248:5504].reshape(4, 64); b3 = P[5504:5508]
+    h1 = np.maximum(x.astype(np.float64) @ W1.T + b1, 0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    return h2 @ W3.T + b3
+
+
+def test_mlp_forward_matches_numpy():
+    rng = np.random.default_rng(0)
+    P = rng.standard_normal(O.MLP_PARAMS) * 0.3
+    x = rng.random((33, 16)).astype(np.float32)
+    np.testing.assert_allclose(O.mlp_forward(P, x), _np_mlp(P, x), rtol=1e-12, atol=1e-12)
+
+
+def test_mlp_gradient_matches_finite_differences():
+    """Central differences of the oracle's own loss pin its analytic gradient
+    (every layer's weights and biases are sampled)."""
+    rng = np.random.default_rng(1)
+    P = rng.standard_normal(O.MLP_PARAMS) * 0.2
+    x = rng.random((7, 16)).astype(np.float32)
+    t = rng.dirichlet([1, 1, 1], 7).astype(np.float32)
+    c = rng.random(7).astype(np.float32)
+    L, g = O.mlp_loss_grad(P, x, t, c, 0.7)
+    # the loss at P equals a numpy restatement of the formula
+    y = _np_mlp(P, x)
+    z = y[:, :3]
+    lse = np.log(np.exp(z - z.max(1, keepdims=True)).sum(1)) + z.max(1)
+    ref = np.mean(-(t * (z - lse[:, None])).sum(1) + 0.7 * (y[:, 3] - c) ** 2)
+    assert abs(L - ref) < 1e-12
+    eps = 1e-6
+    for i in list(rng.choice(O.MLP_PARAMS, 40, replace=False)) + [0, 1023, 1024, 1087, 1088, 5183, 5184, 5247,
+                                                                  5248, 5503, 5504, 5507]:
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i] += eps
+        Pm[i] -= eps
+        fd = (O.mlp_loss_grad(Pp, x, t, c, 0.7)[0] - O.mlp_loss_grad(Pm, x, t, c, 0.7)[0]) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+@pytest.mark.parametrize("kt", [np.int32, np.int64])
+@pytest.mark.parametrize("rows,d", [(100, 7), (4096, 4096), (50_000, 300), (1_000_000, 1 << 20)])
+def test_sample_features_match_numpy(kt, rows, d):
+    rng = np.random.default_rng(rows)
+    keys = rng.integers(-d, d, rows).astype(kt)
+    f = O.sample_features(keys, group_bound=0, nops=5, value_bytes=12)
+    ns = min(rows, 4096)
+    stride = 1 if rows <= 4096 else rows // 4096
+    s = keys[np.arange(ns) * stride].astype(np.int64)
+    u, cnt = np.unique(s, return_counts=True)
+    p = cnt / ns
+    f1, f2 = (cnt == 1).sum(), (cnt == 2).sum()
+    chao = len(u) + (f1 * f1 / (2 * f2) if f2 else f1 * (f1 - 1) / 2)
+    gee = np.sqrt(rows / ns) * f1 + len(u) - f1
+    exp = [np.log2(rows + 1) / 32, np.dtype(kt).itemsize / 8, np.log2(len(u) + 1) / 12,
+           np.log2(min(chao, rows) + 1) / 32, np.log2(min(gee, rows) + 1) / 32, np.sort(cnt)[::-1][:8].sum() / ns,
+           -(p * np.log2(p)).sum() / np.log2(ns), f1 / ns, f2 / ns, cnt.max() / ns,
+           np.log2(float(s.max() - s.min() + 1) + 1) / 64, np.log2(rows + 1) / 32, 5 / 16, 12 / 32,
+           (np.diff(s) >= 0).sum() / (ns - 1), ns / 4096]
+    np.testing.assert_allclose(f, np.array(exp, np.float32), rtol=2e-6, atol=1e-7)
