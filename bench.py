@@ -37,6 +37,8 @@ STRATS = {"auto": "groupby", "hash": "hash_groupby", "sort": "sort_groupby", "so
 # Algorithmic HBM bytes per row of one launch of each timed kernel (int32 key,
 # int64 + float32 values): inputs read once; a radix scatter reads and writes
 # the key and both values once (2 x 16 B).
+KERNELS = ("hash_agg", "hash_compact", "hll", "key_range", "radix_hist", "radix_scan", "radix_scatter", "seg_heads",
+           "segreduce", "partition_bounds", "partition_agg", "partition_compact", "fill_zero", "mlp_select")
 KERNEL_BYTES_PER_ROW = {"hash_agg": 16, "partition_agg": 16, "segreduce": 16, "radix_scatter": 32,
                         "radix_hist": 4, "seg_heads": 4, "key_range": 4, "hll": 4, "partition_bounds": 4}
 
@@ -160,8 +162,7 @@ def main():
 
     # ---- dominant kernel roofline --------------------------------------------------
     kernels = {}
-    for name in ("hash_agg", "hash_compact", "hll", "key_range", "radix_hist", "radix_scatter", "seg_heads",
-                 "segreduce", "partition_bounds", "partition_agg"):
+    for name in KERNELS:
         cnt, ms = G.kernel_time(name)
         if cnt:
             kernels[name] = (cnt, ms)

@@ -215,6 +215,7 @@ __global__ void batch_compact_kernel(const BQuery* __restrict__ qs, const BCItem
                                      unsigned long long* metas, unsigned long long* counts, uint32_t* qstatus,
                                      BOut bo, int W)
 {
+    __shared__ unsigned long long sh[17];
     const BCItem c = citems[blockIdx.x];
     const BQuery q = qs[c.q];
     unsigned long long* meta = metas + (size_t)c.q * kMetaWords;
@@ -234,7 +235,7 @@ __global__ void batch_compact_kernel(const BQuery* __restrict__ qs, const BCItem
                 occ = meta[1] != 0;
             }
         }
-        const int64_t idx = dev::wave_claim(&counts[c.q], occ);
+        const int64_t idx = dev::block_claim(&counts[c.q], occ, sh);
         if (occ) {
             if (idx < q.capacity) emit_batch(bo, q, idx, enc, slot + 1);
             else atomicOr(&qstatus[c.q], kStCapacity);

@@ -346,5 +346,32 @@ __device__ __forceinline__ int64_t wave_claim(unsigned long long* counter, bool 
     return (int64_t)(base + __popcll(m & lanemask_lt()));
 }
 
+// Workgroup-aggregated increment of a device counter: one atomic per
+// workgroup (same-address device atomics serialise, so a wave-level claim
+// over millions of waves becomes the bottleneck).  Every thread of the
+// workgroup must call it; it synchronises.  `sh`: 17 words of LDS, at most
+// 16 waves per workgroup.
+__device__ __forceinline__ int64_t block_claim(unsigned long long* counter, bool take, unsigned long long* sh)
+{
+    const uint64_t m = __ballot(take);
+    const int w = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    if (lane_id() == 0) sh[w] = (unsigned long long)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int i = 0; i < nw; ++i) {
+            const unsigned long long c = sh[i];
+            sh[i] = tot;
+            tot += c;
+        }
+        sh[16] = tot ? atomicAdd(counter, tot) : 0ull;
+    }
+    __syncthreads();
+    const int64_t r = (int64_t)(sh[16] + sh[w] + (unsigned long long)__popcll(m & lanemask_lt()));
+    __syncthreads();  // sh is reused by the next call
+    return r;
+}
+
 }  // namespace dev
 }  // namespace eg

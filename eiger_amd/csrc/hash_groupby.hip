@@ -174,6 +174,7 @@ __global__ __launch_bounds__(max_block<VL::NV>()) void hash_agg_kernel(KeyDesc k
 
 __global__ void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter)
 {
+    __shared__ unsigned long long sh[17];
     const uint64_t total = gt.mask + 2;  // slots + side slot
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
@@ -190,7 +191,7 @@ __global__ void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* c
             enc = 0;
             occ = gt.meta[1] != 0;
         }
-        int64_t idx = dev::wave_claim(counter, occ);
+        const int64_t idx = dev::block_claim(counter, occ, sh);
         if (occ) dev::emit_group(os, idx, enc, slot + 1, 1);
     }
 }

@@ -83,16 +83,25 @@ __device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* red)
     return t;
 }
 
-// 256 threads per query.
-__global__ __launch_bounds__(256) void features_kernel(const FQuery* __restrict__ qs, float* __restrict__ out)
+struct FeatLds {
+    unsigned long long skey[kFSlots];
+    uint32_t scnt[kFSlots];
+    float redf[4];
+    uint32_t redu[4];
+    uint32_t s_asc;
+    unsigned long long s_min, s_max;
+};
+
+// The 16 features of query q, written by thread 0 to o (256 threads).
+__device__ void compute_features(const FQuery& q, FeatLds& L, float* o)
 {
-    __shared__ unsigned long long skey[kFSlots];
-    __shared__ uint32_t scnt[kFSlots];
-    __shared__ float redf[4];
-    __shared__ uint32_t redu[4];
-    __shared__ uint32_t s_asc;
-    __shared__ unsigned long long s_min, s_max;
-    const FQuery q = qs[blockIdx.x];
+    unsigned long long* skey = L.skey;
+    uint32_t* scnt = L.scnt;
+    float* redf = L.redf;
+    uint32_t* redu = L.redu;
+    uint32_t& s_asc = L.s_asc;
+    unsigned long long& s_min = L.s_min;
+    unsigned long long& s_max = L.s_max;
     const int t = threadIdx.x;
     for (int i = t; i < kFSlots; i += blockDim.x) {
         skey[i] = 0ull;
@@ -156,6 +165,7 @@ __global__ __launch_bounds__(256) void features_kernel(const FQuery* __restrict_
     f1 = block_sum(f1, redf);
     f2 = block_sum(f2, redf);
     ent = block_sum(ent, redf);
+    mx = block_max_u32(mx, redu);
     // top-8 frequency mass: eight rounds of a block max over (count, slot)
     // packed as count << 13 | (8191 - slot), removing the maximum each round
     float top8 = 0.f;
@@ -180,7 +190,6 @@ __global__ __launch_bounds__(256) void features_kernel(const FQuery* __restrict_
     const float gee = (ns > 0 ? sqrtf(rows / nsf) : 0.f) * f1 + (ndv - f1);
     const float span = ns > 0 ? (float)(s_max - s_min) + 1.f : 0.f;
     const int64_t bnd = (q.group_bound > 0 && q.group_bound < n) ? q.group_bound : n;
-    float* o = out + (size_t)blockIdx.x * kIn;
     o[0] = log2f(rows + 1.f) / 32.f;
     o[1] = (float)q.key_bytes / 8.f;
     o[2] = log2f(ndv + 1.f) / 12.f;
@@ -197,6 +206,42 @@ __global__ __launch_bounds__(256) void features_kernel(const FQuery* __restrict_
     o[13] = (float)q.value_bytes / 32.f;
     o[14] = ns > 1 ? (float)s_asc / (nsf - 1.f) : 1.f;
     o[15] = nsf / (float)kSample;
+}
+
+// 256 threads per query.
+__global__ __launch_bounds__(256) void features_kernel(const FQuery* __restrict__ qs, float* __restrict__ out)
+{
+    __shared__ FeatLds L;
+    const FQuery q = qs[blockIdx.x];
+    compute_features(q, L, out + (size_t)blockIdx.x * kIn);
+}
+
+// Features of one query and the MLP's four outputs in one launch (the
+// dispatcher's path: one kernel, one 16-byte copy back).
+__global__ __launch_bounds__(256) void select_kernel(FQuery q, const float* __restrict__ P, float* __restrict__ y)
+{
+    __shared__ FeatLds L;
+    __shared__ float x[kIn], h1[kH], h2[kH];
+    compute_features(q, L, x);
+    __syncthreads();
+    const int u = threadIdx.x;
+    if (u < kH) {
+        float z = P[kB1 + u];
+        for (int f = 0; f < kIn; ++f) z = fmaf(P[kW1 + u * kIn + f], x[f], z);
+        h1[u] = z > 0.f ? z : 0.f;
+    }
+    __syncthreads();
+    if (u < kH) {
+        float z = P[kB2 + u];
+        for (int v = 0; v < kH; ++v) z = fmaf(P[kW2 + u * kH + v], h1[v], z);
+        h2[u] = z > 0.f ? z : 0.f;
+    }
+    __syncthreads();
+    if (u < kOut) {
+        float z = P[kB3 + u];
+        for (int v = 0; v < kH; ++v) z = fmaf(P[kW3 + u * kH + v], h2[v], z);
+        y[u] = z;
+    }
 }
 
 // ---- MLP kernels (256 threads, 64 samples per workgroup) ---------------------------
@@ -424,6 +469,8 @@ void mlp_features(const FQuery* dq, int64_t nq, float* dout, hipStream_t s)
 struct LoadedModel {
     std::mutex mu;
     DevBuf params;
+    DevBuf out;                 // 4 floats
+    float* host_out = nullptr;  // pinned, 4 floats
     bool loaded = false;
 };
 LoadedModel& loaded_model()
@@ -440,15 +487,16 @@ bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_
     std::lock_guard<std::mutex> lk(m.mu);
     if (!m.loaded) return false;
     FQuery q{keys, rows, group_bound, type_size(key_type), nops, value_bytes, 0};
-    DevBuf dq(sizeof q, s), feat(kIn * 4, s), out(kOut * 4, s);
-    EG_HIP(hipMemcpyAsync(dq.get(), &q, sizeof q, hipMemcpyHostToDevice, s));
-    mlp_features(dq.get<FQuery>(), 1, feat.get<float>(), s);
-    hipLaunchKernelGGL(mlp_fwd_kernel, dim3(1), dim3(kMB), 0, s, (const float*)m.params.get<float>(),
-                       (const float*)feat.get<float>(), 1, out.get<float>());
-    EG_LAUNCH_CHECK();
-    float h[kOut];
-    EG_HIP(hipMemcpyAsync(h, out.get(), sizeof h, hipMemcpyDeviceToHost, s));
+    {
+        KernelTimer timer("mlp_select", s);
+        hipLaunchKernelGGL(select_kernel, dim3(1), dim3(256), 0, s, q, (const float*)m.params.get<float>(),
+                           m.out.get<float>());
+        EG_LAUNCH_CHECK();
+    }
+    EG_HIP(hipMemcpyAsync(m.host_out, m.out.get(), kOut * 4, hipMemcpyDeviceToHost, s));
     EG_HIP(hipStreamSynchronize(s));
+    float h[kOut];
+    std::memcpy(h, m.host_out, sizeof h);
     for (float v : h)
         if (!std::isfinite(v)) return false;  // fall back to the rule
     const int best = (h[1] > h[0] && h[1] >= h[2]) ? 1 : (h[2] > h[0] && h[2] > h[1]) ? 2 : 0;
@@ -600,6 +648,8 @@ int eg_mlp_load(const float* host_params, int64_t n)
             if (!std::isfinite(host_params[i])) eg::raise(EG_ERR_INVALID, "non-finite parameter");
         m.params = eg::DevBuf((size_t)n * 4, nullptr);
         EG_HIP(hipMemcpy(m.params.get(), host_params, (size_t)n * 4, hipMemcpyHostToDevice));
+        m.out = eg::DevBuf(eg::kOut * 4, nullptr);
+        if (!m.host_out) EG_HIP(hipHostMalloc((void**)&m.host_out, eg::kOut * 4, hipHostMallocDefault));
         m.loaded = true;
     });
 }

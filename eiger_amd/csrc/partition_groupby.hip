@@ -73,6 +73,7 @@ __global__ __launch_bounds__(kPB) void part_agg_kernel(KeyDesc key, AggSpec spec
 {
     using KT = typename LdsKey<KB>::T;
     extern __shared__ unsigned long long smem[];
+    __shared__ unsigned long long sh[17];
     const int nops = spec.nops;
     const int lane = dev::lane_id();
     LdsTable<KB> tab(smem, nops, cr_log2, 0);
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(kPB) void part_agg_kernel(KeyDesc key, AggSpec spec
                                       [&](int a) { return tab.st[(size_t)a * tab.T + s]; });
             }
         } else {
-            const int64_t idx = dev::wave_claim(counter, occ);
+            const int64_t idx = dev::block_claim(counter, occ, sh);
             if (occ) {
                 if (idx < capacity) dev::emit_group(os, idx, enc, tab.st + s, (int64_t)tab.T);
                 else atomicExch(&gt.meta[2], 1ull);  // output capacity exceeded: retry larger
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(kPB) void part_agg_kernel(KeyDesc key, AggSpec spec
 // Appends the HBM table's groups after the directly written ones.
 __global__ void part_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter, int64_t capacity)
 {
+    __shared__ unsigned long long sh[17];
     const uint64_t total = gt.mask + 2;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
@@ -145,7 +147,7 @@ __global__ void part_compact_kernel(GTable gt, OutSpec os, unsigned long long* c
             slot = gt.side;
             occ = gt.meta[1] != 0;
         }
-        const int64_t idx = dev::wave_claim(counter, occ);
+        const int64_t idx = dev::block_claim(counter, occ, sh);
         if (occ && idx < capacity) dev::emit_group(os, idx, enc, slot + 1, 1);
     }
 }
@@ -167,7 +169,7 @@ void launch_part_agg(const KeyDesc& key, const AggSpec& spec, const int64_t* per
         if (enable_dynamic_lds((const void*)kern) < lds) raise(EG_ERR_UNSUPPORTED, "partition LDS table too large");
         KernelTimer timer("partition_agg", s);
         hipLaunchKernelGGL(kern, dim3(nunits), dim3(kPB), lds, s, key, spec, perm, units, gt, os, counter, capacity,
-                           cr_log2, 16, 8);
+                           cr_log2, 64, 8);
         EG_LAUNCH_CHECK();
     };
     if (nv <= 1) go(part_agg_kernel<KB, 1>);
@@ -278,7 +280,13 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     std::vector<int64_t> ho((size_t)(P + 1));
     EG_HIP(hipMemcpyAsync(ho.data(), offs.get(), (size_t)(P + 1) * 8, hipMemcpyDeviceToHost, s));
     EG_HIP(hipStreamSynchronize(s));
-    const int64_t slice = std::max<int64_t>((int64_t)kPB * 16, 8 * per_part);  // rows per unit of a large partition
+    // Rows per unit: a partition is split when it is much larger than the
+    // average (skew), or when there are too few partitions to occupy every CU,
+    // since a split partition's groups merge through the HBM table with
+    // atomics instead of being written straight out.
+    const int64_t avg_rows = (n + P - 1) / P;
+    const int64_t balance = std::max<int64_t>(1, n / (4 * (int64_t)di.num_cus));
+    const int64_t slice = std::max<int64_t>({(int64_t)kPB * 16, 8 * per_part, std::min(4 * avg_rows, balance)});
     std::vector<Unit> units;
     for (int64_t p = 0; p < P; ++p) {
         const int64_t a = ho[(size_t)p], e = ho[(size_t)p + 1];
@@ -299,7 +307,10 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     for (int64_t p = 0; p < P; ++p) sliced_parts += (ho[(size_t)p + 1] - ho[(size_t)p]) > slice ? 1 : 0;
     int64_t capacity = bound;
     int64_t slots = 1 << 14;
-    while (slots < 4 * std::min<int64_t>(bound, sliced_parts * per_part * 2)) slots <<= 1;
+    // sliced partitions' groups, plus keys the LDS tables may not hold (a
+    // partition above its expected size, long probe runs): 1/32 of the bound
+    const int64_t spill_groups = std::min<int64_t>(bound, sliced_parts * per_part * 2) + bound / 32;
+    while (slots < 2 * spill_groups) slots <<= 1;
     const int W = table_words_per_slot(nops);
     for (;;) {
         Result res;
@@ -321,6 +332,7 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
             // append HBM-table groups
             const uint64_t total = ht.gt.mask + 2;
             const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + 255) / 256), (int64_t)di.num_cus * 8));
+            KernelTimer timer("partition_compact", s);
             hipLaunchKernelGGL(part_compact_kernel, dim3((unsigned)grid), dim3(256), 0, s, ht.gt, os,
                                counter.get<unsigned long long>(), capacity);
             EG_LAUNCH_CHECK();

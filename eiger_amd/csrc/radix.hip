@@ -7,12 +7,13 @@
 //   radix_hist_kernel    per 4096-row tile, an LDS histogram of the digit;
 //   scan                 exclusive prefix over [digit][tile] (device-wide);
 //   radix_scatter_kernel per tile, a stable rank of every row among the
-//                        tile's rows of the same digit (wave64 ballot match on
-//                        the digit bits, then a prefix over the 4 waves and
-//                        the 16 rounds), the tile rearranged in LDS so rows of
-//                        one digit are contiguous, then written out in runs
-//                        (coalesced stores).  Payload columns follow the same
-//                        permutation, one column at a time through LDS.
+//                        tile's rows of the same digit (each wave ranks its own
+//                        1024 consecutive rows with wave64 ballots on the digit
+//                        bits and running per-wave counters, then one prefix
+//                        over waves and digits), the tile rearranged in LDS so
+//                        rows of one digit are contiguous, then written out in
+//                        runs (coalesced stores).  Payload columns follow the
+//                        same permutation, one column at a time through LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -81,68 +82,82 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wsu
     return x - v + add;
 }
 
+// Stable multisplit of one 4096-row tile.  Wave w owns rows
+// [w*1024, (w+1)*1024) of the tile and walks them 64 at a time: the lanes
+// with the same digit are found with one ballot per digit bit, the lowest of
+// them advances the wave's running count of that digit in LDS.  No barrier is
+// needed inside the walk (each wave updates only its own counters); one scan
+// over (wave, digit) then gives every row its place in the tile.
 __global__ __launch_bounds__(kRB) void radix_scatter_kernel(RadixKeys k, void* __restrict__ kout, PayArgs pay,
                                                              int64_t n, int shift, int bits,
                                                              const uint32_t* __restrict__ offs, int ntiles)
 {
+    constexpr int kW = kRB / 64;   // waves
+    constexpr int kWR = kRT / kW;  // rows per wave
+    static_assert(kRB == 256 && kWR == kRI * 64, "one digit per thread; kRI rows per lane");
     __shared__ unsigned long long stage[kRT];
     __shared__ uint8_t sdig[kRT];
-    __shared__ uint32_t wcnt[4][256];
-    __shared__ uint32_t woff[4][256];
-    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t whist[kW][256];
     __shared__ uint32_t lstart[256];
     __shared__ uint32_t gofs[256];
-    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wsum[kW];
 
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const uint32_t mask = (1u << bits) - 1u;
     const int64_t tbase = (int64_t)blockIdx.x * kRT;
-    cnt[t] = 0u;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) wcnt[w][t] = 0u;
+    for (int w = 0; w < kW; ++w) whist[w][t] = 0u;
     gofs[t] = offs[(int64_t)t * ntiles + blockIdx.x];
     __syncthreads();
 
-    uint32_t dg[kRI], pos[kRI];
+    uint32_t dg[kRI], rk[kRI];
     const uint64_t lt = dev::lanemask_lt();
+    const int64_t wbase = tbase + (int64_t)wave * kWR;
 #pragma unroll
     for (int i = 0; i < kRI; ++i) {
-        const int64_t e = tbase + (int64_t)i * kRB + t;
+        const int64_t e = wbase + (int64_t)i * 64 + lane;
         const bool valid = e < n;
-        dg[i] = valid ? digit_of(k, dev::load_raw(k.data, k.bytes, e), shift, mask) : 0u;
+        const uint32_t d = valid ? digit_of(k, dev::load_raw(k.data, k.bytes, e), shift, mask) : 0u;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < bits; ++b) {
-            const bool bit = (dg[i] >> b) & 1u;
+            const bool bit = (d >> b) & 1u;
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        const uint32_t rank = (uint32_t)__popcll(peers & lt);
-        if (valid && rank == 0u) wcnt[wave][dg[i]] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {
-            uint32_t run = cnt[t];
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                const uint32_t c = wcnt[w][t];
-                woff[w][t] = run;
-                run += c;
-                wcnt[w][t] = 0u;
-            }
-            cnt[t] = run;
-        }
-        __syncthreads();
-        pos[i] = valid ? woff[wave][dg[i]] + rank : 0xffffffffu;
+        const uint32_t before =
+            valid ? __hip_atomic_load(&whist[wave][d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0u;
+        const uint32_t r = (uint32_t)__popcll(peers & lt);
+        // every lane of the wave has read the counter before the leader writes it
+        __builtin_amdgcn_wave_barrier();
+        if (valid && r == 0u)
+            __hip_atomic_store(&whist[wave][d], before + (uint32_t)__popcll(peers), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __builtin_amdgcn_wave_barrier();
+        dg[i] = d;
+        rk[i] = valid ? before + r : 0xffffffffu;
     }
-    const uint32_t ls = block_excl_scan256(cnt[t], wsum);
+    __syncthreads();
+    // per digit: exclusive prefix over the waves, and the digit's tile count
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kW; ++w) {
+        const uint32_t c = whist[w][t];
+        whist[w][t] = run;
+        run += c;
+    }
+    const uint32_t ls = block_excl_scan256(run, wsum);
     lstart[t] = ls;
     __syncthreads();
     const int32_t tcount = (int32_t)min<int64_t>((int64_t)kRT, n - tbase);
+    uint32_t pos[kRI];
 #pragma unroll
-    for (int i = 0; i < kRI; ++i)
-        if (pos[i] != 0xffffffffu) {
-            pos[i] += lstart[dg[i]];
+    for (int i = 0; i < kRI; ++i) {
+        pos[i] = 0xffffffffu;
+        if (rk[i] != 0xffffffffu) {
+            pos[i] = lstart[dg[i]] + whist[wave][dg[i]] + rk[i];
             sdig[pos[i]] = (uint8_t)dg[i];
         }
+    }
 
     // keys, then every payload column, through the LDS stage
     for (int c = -1; c < pay.n; ++c) {
@@ -154,7 +169,7 @@ __global__ __launch_bounds__(kRB) void radix_scatter_kernel(RadixKeys k, void* _
 #pragma unroll
         for (int i = 0; i < kRI; ++i)
             if (pos[i] != 0xffffffffu) {
-                const int64_t e = tbase + (int64_t)i * kRB + t;
+                const int64_t e = wbase + (int64_t)i * 64 + lane;
                 stage[pos[i]] = rid ? (unsigned long long)e : (unsigned long long)dev::load_raw(src, bytes, e);
             }
         __syncthreads();
@@ -317,7 +332,10 @@ void radix_pass(const RadixKeys& k, void* keys_out, const std::vector<Payload>& 
                            ntiles);
         EG_LAUNCH_CHECK();
     }
-    scan_exclusive_u32(hist.get<uint32_t>(), offs.get<uint32_t>(), (int64_t)256 * ntiles, s, false);
+    {
+        KernelTimer timer("radix_scan", s);
+        scan_exclusive_u32(hist.get<uint32_t>(), offs.get<uint32_t>(), (int64_t)256 * ntiles, s, false);
+    }
     PayArgs pa{};
     pa.n = (int32_t)pay.size();
     for (size_t c = 0; c < pay.size(); ++c) {

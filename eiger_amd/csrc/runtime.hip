@@ -67,6 +67,7 @@ void fill_zero(void* p, size_t bytes, hipStream_t s)
     int64_t n16 = (int64_t)(bytes / 16);
     int64_t ntail = (int64_t)(bytes - (size_t)n16 * 16);
     int64_t blocks = std::min<int64_t>(std::max<int64_t>((n16 + 255) / 256, 1), 8192);
+    KernelTimer timer("fill_zero", s);
     hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)p, n16,
                        (uint8_t*)p + n16 * 16, ntail);
     EG_LAUNCH_CHECK();

@@ -109,8 +109,8 @@ def trace(specs, seed0: int, wl: Workload, progress_every: int = 500):
     t0 = time.time()
     for i, spec in enumerate(specs):
         k, vals, reqs, nops, vb = wl.query(spec, seed0 + i)
-        f, feat_ms[i] = timed(G.mlp_features, [k], None, nops, vb)
-        feats[i] = f[0]
+        feats[i] = G.mlp_features([k], None, nops, vb)[0]
+        _, feat_ms[i] = timed(G.mlp_select, k, 0, nops, vb)  # the dispatcher's own cost
         h = G.groupby_hint()
         _, rule_ms[i] = timed(G.auto_select_groupby_algo, [k], h)
         rule[i] = ALGO_INDEX.get(h.groupby_algo, 0)
@@ -138,7 +138,7 @@ def evaluate(params: np.ndarray, tr: dict, infer_ms: float) -> dict:
     y = G.mlp_forward(params, x)
     pick = np.argmax(y[:, :3], axis=1)
     best = t.min(axis=1)
-    mlp_total = float(t[np.arange(n), pick].sum() + tr["feat_ms"].sum() + infer_ms * n)
+    mlp_total = float(t[np.arange(n), pick].sum() + tr["feat_ms"].sum())
     rule_total = float(t[np.arange(n), tr["rule"]].sum() + tr["rule_ms"].sum())
     out = {
         "queries": n,
@@ -147,7 +147,7 @@ def evaluate(params: np.ndarray, tr: dict, infer_ms: float) -> dict:
         "rule_ms": rule_total,
         "mlp_choice_only_ms": float(t[np.arange(n), pick].sum()),
         "rule_choice_only_ms": float(t[np.arange(n), tr["rule"]].sum()),
-        "mlp_overhead_ms": float(tr["feat_ms"].sum() + infer_ms * n),
+        "mlp_overhead_ms": float(tr["feat_ms"].sum()),
         "rule_overhead_ms": float(tr["rule_ms"].sum()),
         "fixed_ms": {s: float(t[:, j].sum()) for j, s in enumerate(STRATS)},
         "mlp_over_best": mlp_total / float(best.sum()),
@@ -179,6 +179,7 @@ def main():
         for name in STRATS:
             FNS[name]([k], vals, reqs, G.groupby_hint(groups_upper_bound=spec[2])).free()
         k.free()
+    G.mlp_load(G.mlp_init(0))  # any weights: the trace times the dispatcher's select path
     log("tracing training queries")
     tr = trace(specs_tr, 1_000_000, wl)
     log("tracing held-out queries")
@@ -188,17 +189,14 @@ def main():
     card = (np.log2(np.maximum(tr["groups"], 1)) / 32).astype(np.float32)
     P0 = G.mlp_init(0)
     a = time.perf_counter()
-    params, losses = G.mlp_train(P0, tr["features"], target, card, epochs=args.epochs, batch=1024, lr=2e-3,
+    batch = max(64, min(1024, (len(target) // 64) * 64))
+    params, losses = G.mlp_train(P0, tr["features"], target, card, epochs=args.epochs, batch=batch, lr=2e-3,
                                  lam=4.0, weight_decay=1e-5, seed=3)
     train_s = time.perf_counter() - a
     log(f"trained {args.epochs} epochs in {train_s:.1f}s: loss {losses[0]:.4f} -> {losses[-1]:.4f}")
 
-    # inference cost per query: features are timed in the trace; forward of one sample here
-    G.mlp_forward(params, ho["features"][:1])
-    a = time.perf_counter()
-    for i in range(50):
-        G.mlp_forward(params, ho["features"][i:i + 1])
-    infer_ms = (time.perf_counter() - a) * 1e3 / 50
+    G.mlp_load(None)
+    infer_ms = float(np.median(ho["feat_ms"]))  # features + inference per query (select path)
 
     ev = {"train": evaluate(params, tr, infer_ms), "heldout": evaluate(params, ho, infer_ms),
           "epochs": args.epochs, "tau": args.tau, "train_seconds": train_s,
