@@ -55,6 +55,7 @@ def parse():
     p.add_argument("--rows", type=int, default=1 << 28)
     p.add_argument("--strategy", default="auto", choices=list(STRATS))
     p.add_argument("--explore", action="store_true")
+    p.add_argument("--rule-only", action="store_true", help="auto dispatch without the learned model")
     p.add_argument("--cases", default="all", help="'all' or comma list like 16:uniform,1024:zipf1.0")
     p.add_argument("--cpu-sample-rows", type=int, default=1 << 22)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -133,6 +134,26 @@ def main():
                 run_case(i, f)
                 explore.setdefault(i, {})[name] = round((time.perf_counter() - a) * 1e3, 3)
             log(f"explored {name}")
+    # ---- dispatch: the learned model (when shipped) against the reference rule ------
+    dispatch = None
+    learned = args.strategy == "auto" and not args.rule_only and G.load_dispatch_model()
+    if args.strategy == "auto":
+        G.mlp_load(None)  # the rule of auto_select_groupby_algo
+        for i in range(len(cases)):
+            run_case(i)
+        rule_t = time.perf_counter()
+        for k in range(args.steps):
+            for i in range(len(cases)):
+                run_case(i)
+        rule_elapsed = time.perf_counter() - rule_t
+        rule_value = n * len(cases) * args.steps / rule_elapsed / 1e9
+        log(f"rule dispatch: {rule_value:.3f} Grows/s")
+        if learned:
+            G.load_dispatch_model()
+            for i in range(len(cases)):
+                run_case(i)
+        dispatch = {"rule_grows_per_s": round(rule_value, 3), "learned": bool(learned)}
+
     G.kernel_timing_reset()
     G.kernel_timing(True)
     per_case = [0.0] * len(cases)
@@ -145,6 +166,16 @@ def main():
         log(f"step {k + 1}/{args.steps} done")
     elapsed = time.perf_counter() - t_start
     G.kernel_timing(False)
+    if dispatch is not None:
+        v = n * len(cases) * args.steps / elapsed / 1e9
+        dispatch["learned_grows_per_s" if learned else "timed_grows_per_s"] = round(v, 3)
+        if learned:
+            dispatch["learned_vs_rule"] = round(v / dispatch["rule_grows_per_s"], 3)
+            ev = os.path.join(ROOT, "eiger_amd", "data", "dispatch_eval.json")
+            if os.path.exists(ev):
+                e = json.load(open(ev)).get("heldout_batched", {})
+                dispatch["heldout_mixed_queries"] = {k: e.get(k) for k in (
+                    "queries", "mlp_over_best", "rule_over_mlp", "mlp_picks_best", "rule_picks_best")}
 
     rows = n * len(cases) * args.steps
     value = rows / elapsed / 1e9
@@ -194,10 +225,13 @@ def main():
         "config": {"workload": "single-query strategy sweep: 2^28-row int32-key group-by, D in "
                                "{2^4,2^10,2^16,2^22,2^26} x {uniform, zipf 0.5/1.0/1.5}; COUNT, "
                                "SUM/MIN/MAX(int64), SUM(float32)",
-                   "rows": n, "cases": len(cases), "strategy": args.strategy,
+                   "rows": n, "cases": len(cases),
+                   "strategy": ("auto: learned dispatch" if dispatch and dispatch.get("learned") else
+                                "auto: rule dispatch" if args.strategy == "auto" else args.strategy),
                    "hbm_frac_algorithmic": round(alg_bytes / elapsed / PEAK_HBM, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "dispatch": dispatch,
         "sweep": sweep,
     }
     print(json.dumps(out), flush=True)

@@ -126,6 +126,8 @@ def _proto(L):
     f("eg_mlp_train", C.c_int, [vp, vp, vp, vp, i64, C.POINTER(MlpTrainConfig), PF, vp])
     f("eg_mlp_load", C.c_int, [vp, i64])
     f("eg_mlp_loaded", C.c_int, [])
+    f("eg_mlp_select_batch", C.c_int, [PMQ, i64, vp, vp, vp])
+    f("eg_auto_select_algo_batch", C.c_int, [PCV, i64, vp, vp, vp])
     f("eg_mlp_select", C.c_int, [ColumnView, i64, i32, i32, vp, C.POINTER(i32), C.POINTER(i64)])
     PBQ, PBT = C.POINTER(BatchQuery), C.POINTER(BatchTuning)
     f("eg_batch_tuning_default", None, [PBT])

@@ -365,6 +365,7 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
         }
         if (algo == EG_ALGO_SORT_OPT) h.optimize_sort = 1;
 
+        HostPhase phase_all("host_groupby_call", s);
         Result res;
         if (plan.rows == 0) {
             OutSpec os;

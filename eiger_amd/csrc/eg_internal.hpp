@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+
 #include <cstdint>
 #include <memory>
 #include <new>
@@ -138,6 +140,20 @@ private:
     const char* name_;
     hipStream_t s_;
     hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
+// Wall time of a host-orchestrated phase (synchronises the stream at both
+// ends, only while timing is enabled); recorded under `name`.
+class HostPhase {
+public:
+    HostPhase(const char* name, hipStream_t s);
+    ~HostPhase();
+
+private:
+    const char* name_;
+    hipStream_t s_;
+    bool on_ = false;
+    std::chrono::steady_clock::time_point t0_;
 };
 
 // ---- fills ------------------------------------------------------------------------

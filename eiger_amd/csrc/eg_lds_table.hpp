@@ -67,6 +67,78 @@ struct VLayout {
     static constexpr int conv() { return C == 0 ? C0_ : C1_; }
 };
 
+// Column kinds of the compiled layouts: 0 = 8-byte, 1 = int32, 2 = uint32,
+// 3 = float32.
+constexpr int kind_width(int k) { return k == 0 ? 8 : 4; }
+constexpr int kind_conv(int k) { return k == 1 ? CONV_SEXT32 : k == 3 ? CONV_F32 : CONV_RAW; }
+
+template <class F>
+void with_kind(int k, F&& f)
+{
+    switch (k) {
+    case 0: f(std::integral_constant<int, 0>{}); break;
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    default: f(std::integral_constant<int, 3>{}); break;
+    }
+}
+
+// Calls f(VLayout{}) with the compiled layout of spec's value columns: fixed
+// widths and conversions for up to two 4/8-byte columns (when `fixed_ok`),
+// the generic typed-load layout otherwise.
+template <class F>
+void with_vlayout(const AggSpec& spec, bool fixed_ok, F&& f)
+{
+    const int nv = spec.nvcols;
+    bool fast = fixed_ok && nv <= 2;
+    for (int c = 0; c < nv; ++c) fast = fast && (spec.cols[c].bytes == 4 || spec.cols[c].bytes == 8);
+    if (fast) {
+        auto kind = [&](int c) {
+            const ColDesc& d = spec.cols[c];
+            return d.bytes == 8 ? 0 : d.conv == CONV_SEXT32 ? 1 : d.conv == CONV_F32 ? 3 : 2;
+        };
+        if (nv == 0) return f(VLayout<0, 4, 4, false>{});
+        if (nv == 1)
+            return with_kind(kind(0), [&](auto k0) {
+                constexpr int K0 = decltype(k0)::value;
+                f(VLayout<1, kind_width(K0), 4, false, kind_conv(K0)>{});
+            });
+        return with_kind(kind(0), [&](auto k0) {
+            with_kind(kind(1), [&](auto k1) {
+                constexpr int K0 = decltype(k0)::value, K1 = decltype(k1)::value;
+                f(VLayout<2, kind_width(K0), kind_width(K1), false, kind_conv(K0), kind_conv(K1)>{});
+            });
+        });
+    }
+    if (nv <= 2) return f(VLayout<2, 0, 0, true>{});
+    if (nv <= 4) return f(VLayout<4, 0, 0, true>{});
+    return f(VLayout<8, 0, 0, true>{});
+}
+
+// Values of row r (typed loads; one coalesced element per lane).
+template <class VL>
+__device__ __forceinline__ void load_row(const AggSpec& spec, int64_t r, bool valid,
+                                         uint64_t (&v)[VL::NV > 0 ? VL::NV : 1])
+{
+    if constexpr (VL::GEN) {
+#pragma unroll
+        for (int c = 0; c < VL::NV; ++c)
+            v[c] = (valid && c < spec.nvcols) ? dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, r) : 0ull;
+    } else {
+        if constexpr (VL::NV == 0) v[0] = 0ull;
+        if constexpr (VL::NV >= 1) {
+            constexpr int W = VL::template width<0>();
+            v[0] = !valid ? 0ull : W == 4 ? (uint64_t)((const uint32_t*)spec.cols[0].data)[r]
+                                          : (uint64_t)((const uint64_t*)spec.cols[0].data)[r];
+        }
+        if constexpr (VL::NV >= 2) {
+            constexpr int W = VL::template width<1>();
+            v[1] = !valid ? 0ull : W == 4 ? (uint64_t)((const uint32_t*)spec.cols[1].data)[r]
+                                          : (uint64_t)((const uint64_t*)spec.cols[1].data)[r];
+        }
+    }
+}
+
 template <class VL>
 __device__ __forceinline__ void load_tile(const KeyDesc& key, int kb, const AggSpec& spec, int64_t r0,
                                           uint64_t (&k)[kRows], uint64_t (&v)[VL::NV > 0 ? VL::NV : 1][kRows])

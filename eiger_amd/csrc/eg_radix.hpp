@@ -40,7 +40,8 @@ struct Payload {
 void radix_pass(const RadixKeys& k, void* keys_out, const std::vector<Payload>& pay, int64_t n, int shift,
                 int bits, hipStream_t s);
 
-// Stable LSD sort over digit bits [begin_bit, end_bit) in 8-bit passes.
+// Stable LSD sort over digit bits [begin_bit, end_bit) in balanced passes of
+// at most 6 bits.
 // Returns the buffers holding the result (the last pass writes the *_out
 // buffers when the pass count is odd, the *_alt buffers otherwise; the
 // first pass reads the inputs).

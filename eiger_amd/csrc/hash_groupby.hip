@@ -217,53 +217,13 @@ void launch_agg(const DeviceInfo& di, const HashTuning& t, const KeyDesc& key, i
     EG_LAUNCH_CHECK();
 }
 
-constexpr int kind_width(int k) { return k == 0 ? 8 : 4; }
-constexpr int kind_conv(int k) { return k == 1 ? CONV_SEXT32 : k == 3 ? CONV_F32 : CONV_RAW; }
-
-template <class F>
-void with_kind(int k, F&& f)
-{
-    switch (k) {
-    case 0: f(std::integral_constant<int, 0>{}); break;
-    case 1: f(std::integral_constant<int, 1>{}); break;
-    case 2: f(std::integral_constant<int, 2>{}); break;
-    default: f(std::integral_constant<int, 3>{}); break;
-    }
-}
-
 // Picks the compiled value layout: vector loads for up to two 4/8-byte
 // columns on 16-byte aligned data, the generic typed-load kernel otherwise.
 template <int KB>
 void dispatch_layout(bool vec, const DeviceInfo& di, const HashTuning& t, const KeyDesc& key, int64_t n,
                      const AggSpec& spec, const GTable& gt, const HashLayout& L, hipStream_t s)
 {
-    const int nv = spec.nvcols;
-    auto w = [&](int c) { return spec.cols[c].bytes; };
-    bool fast = vec && nv <= 2;
-    for (int c = 0; c < nv; ++c) fast = fast && (w(c) == 4 || w(c) == 8);
-    if (fast) {
-        // column kind: 0 = 8-byte, 1 = int32, 2 = uint32, 3 = float32
-        auto kind = [&](int c) {
-            const ColDesc& d = spec.cols[c];
-            return d.bytes == 8 ? 0 : d.conv == CONV_SEXT32 ? 1 : d.conv == CONV_F32 ? 3 : 2;
-        };
-        if (nv == 0) return launch_agg<KB, VLayout<0, 4, 4, false>>(di, t, key, n, spec, gt, L, s);
-        if (nv == 1)
-            return with_kind(kind(0), [&](auto k0) {
-                constexpr int K0 = decltype(k0)::value;
-                launch_agg<KB, VLayout<1, kind_width(K0), 4, false, kind_conv(K0)>>(di, t, key, n, spec, gt, L, s);
-            });
-        return with_kind(kind(0), [&](auto k0) {
-            with_kind(kind(1), [&](auto k1) {
-                constexpr int K0 = decltype(k0)::value, K1 = decltype(k1)::value;
-                launch_agg<KB, VLayout<2, kind_width(K0), kind_width(K1), false, kind_conv(K0), kind_conv(K1)>>(
-                    di, t, key, n, spec, gt, L, s);
-            });
-        });
-    }
-    if (nv <= 2) return launch_agg<KB, VLayout<2, 0, 0, true>>(di, t, key, n, spec, gt, L, s);
-    if (nv <= 4) return launch_agg<KB, VLayout<4, 0, 0, true>>(di, t, key, n, spec, gt, L, s);
-    return launch_agg<KB, VLayout<8, 0, 0, true>>(di, t, key, n, spec, gt, L, s);
+    with_vlayout(spec, vec, [&](auto vl) { launch_agg<KB, decltype(vl)>(di, t, key, n, spec, gt, L, s); });
 }
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }

@@ -669,6 +669,45 @@ int eg_mlp_select(eg_column_view key, int64_t group_bound, int32_t nops, int32_t
     });
 }
 
+int eg_mlp_select_batch(const eg_mlp_query* queries, int64_t num, int32_t* algos, int64_t* card_est, void* stream)
+{
+    return eg::guard([&] {
+        if (num < 0 || (num > 0 && (!queries || !algos))) eg::raise(EG_ERR_INVALID, "bad arguments");
+        if (num == 0) return;
+        if (num >= ((int64_t)1 << 31)) eg::raise(EG_ERR_UNSUPPORTED, "too many queries");
+        eg::LoadedModel& m = eg::loaded_model();
+        std::lock_guard<std::mutex> lk(m.mu);
+        if (!m.loaded) eg::raise(EG_ERR_INVALID, "no dispatch model is loaded");
+        hipStream_t s = (hipStream_t)stream;
+        std::vector<eg::FQuery> h((size_t)num);
+        for (int64_t i = 0; i < num; ++i) {
+            const eg_mlp_query& u = queries[i];
+            if (!(u.key_type == EG_INT32 || u.key_type == EG_INT64 || u.key_type == EG_UINT32 || u.key_type == EG_UINT64))
+                eg::raise(EG_ERR_UNSUPPORTED, "features need 4- or 8-byte integer keys");
+            if (u.rows < 0 || (u.rows > 0 && !u.keys)) eg::raise(EG_ERR_INVALID, "bad key column");
+            h[(size_t)i] = eg::FQuery{u.keys, u.rows, u.group_bound, eg::type_size(u.key_type), u.nops, u.value_bytes, 0};
+        }
+        eg::DevBuf dq(h.size() * sizeof(eg::FQuery), s), feat((size_t)num * eg::kIn * 4, s), out((size_t)num * eg::kOut * 4, s);
+        EG_HIP(hipMemcpyAsync(dq.get(), h.data(), h.size() * sizeof(eg::FQuery), hipMemcpyHostToDevice, s));
+        eg::mlp_features(dq.get<eg::FQuery>(), num, feat.get<float>(), s);
+        hipLaunchKernelGGL(eg::mlp_fwd_kernel, dim3((unsigned)((num + eg::kS - 1) / eg::kS)), dim3(eg::kMB), 0, s,
+                           (const float*)m.params.get<float>(), (const float*)feat.get<float>(), (int)num, out.get<float>());
+        EG_LAUNCH_CHECK();
+        std::vector<float> y((size_t)num * eg::kOut);
+        EG_HIP(hipMemcpyAsync(y.data(), out.get(), y.size() * 4, hipMemcpyDeviceToHost, s));
+        EG_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < num; ++i) {
+            const float* o = y.data() + (size_t)i * eg::kOut;
+            const int best = (o[1] > o[0] && o[1] >= o[2]) ? 1 : (o[2] > o[0] && o[2] > o[1]) ? 2 : 0;
+            algos[i] = best == 0 ? EG_ALGO_HASH : best == 1 ? EG_ALGO_SORT : EG_ALGO_PARTITION;
+            if (card_est) {
+                const double lg = std::min(62.0, std::max(0.0, (double)o[3] * 32.0));
+                card_est[i] = (int64_t)std::min<double>((double)std::max<int64_t>(queries[i].rows, 1), std::exp2(lg));
+            }
+        }
+    });
+}
+
 int eg_mlp_loaded(void)
 {
     eg::LoadedModel& m = eg::loaded_model();

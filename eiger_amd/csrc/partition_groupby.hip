@@ -23,6 +23,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <cstdlib>
+#include <cstdio>
+#include <memory>
 #include <vector>
 
 #include "eg_atomics.hpp"
@@ -37,7 +40,33 @@ namespace {
 
 using namespace lds;
 
-constexpr int kPB = 1024;  // threads per unit block
+constexpr int kPBMax = 1024;  // largest workgroup of the unit kernel
+
+// Digit width of one partitioning pass.  Fan-out 64 keeps the rows of one
+// digit in runs long enough for full-line stores (measured: a 6-bit pass on
+// 2^28 rows takes about half the time of an 8-bit one).
+constexpr int kMaxDigitBits = 6;
+
+// Workgroup size and LDS table slots (log2) of the unit kernel; overridable
+// for tuning with EG_PARTITION_TUNE="slots_log2,threads".
+struct PartTune {
+    int cr_log2 = 11;
+    int block = 1024;
+    bool fixed = false;  // set by the override
+};
+PartTune part_tune()
+{
+    PartTune t;
+    if (const char* e = std::getenv("EG_PARTITION_TUNE")) {
+        int a = 0, b = 0;
+        if (std::sscanf(e, "%d,%d", &a, &b) == 2 && a >= 6 && a <= 12 && (b == 256 || b == 512 || b == 1024)) {
+            t.cr_log2 = a;
+            t.block = b;
+            t.fixed = true;
+        }
+    }
+    return t;
+}
 
 struct Unit {
     int64_t start;
@@ -65,12 +94,14 @@ __global__ void part_bounds_kernel(const void* keys, int64_t n, uint32_t mask, i
     }
 }
 
-template <int KB, int NV>
-__global__ __launch_bounds__(kPB) void part_agg_kernel(KeyDesc key, AggSpec spec, const int64_t* perm,
+template <int KB, class VL>
+__global__ __launch_bounds__(kPBMax) void part_agg_kernel(KeyDesc key, AggSpec spec, const int64_t* perm,
                                                        const Unit* units, GTable gt, OutSpec os,
                                                        unsigned long long* counter, int64_t capacity,
                                                        int cr_log2, int probe_limit, int peel_min)
 {
+    constexpr int NV = VL::NV > 0 ? VL::NV : 1;
+    constexpr int J = 4;  // rows per lane per step: independent loads in flight
     using KT = typename LdsKey<KB>::T;
     extern __shared__ unsigned long long smem[];
     __shared__ unsigned long long sh[17];
@@ -83,19 +114,24 @@ __global__ __launch_bounds__(kPB) void part_agg_kernel(KeyDesc key, AggSpec spec
 
     const Unit u = units[blockIdx.x];
     uint32_t ins = 0;  // HBM-table insertions by this lane
-    for (int64_t b = u.start; b < u.end; b += blockDim.x) {
-        const int64_t r = b + threadIdx.x;
-        const bool valid = r < u.end;
-        const int64_t src = (perm && valid) ? perm[r] : r;
-        const uint64_t kraw = valid ? dev::load_raw(key.data, KB, r) : 0ull;
-        uint64_t raw[NV], cv[NV];
+    for (int64_t b = u.start; b < u.end; b += (int64_t)J * blockDim.x) {
+        uint64_t kr[J], raw[J][NV];
+        bool valid[J];
 #pragma unroll
-        for (int c = 0; c < NV; ++c)
-            raw[c] = (valid && c < spec.nvcols) ? dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, src) : 0ull;
+        for (int j = 0; j < J; ++j) {
+            const int64_t r = b + (int64_t)j * blockDim.x + threadIdx.x;
+            valid[j] = r < u.end;
+            const int64_t src = (perm && valid[j]) ? perm[r] : r;
+            kr[j] = valid[j] ? (KB == 4 ? (uint64_t)((const uint32_t*)key.data)[r] : ((const uint64_t*)key.data)[r]) : 0ull;
+            load_row<VL>(spec, src, valid[j], raw[j]);
+        }
 #pragma unroll
-        for (int c = 0; c < NV; ++c) cv[c] = c < spec.nvcols ? dev::canon(raw[c], spec.cols[c].conv) : 0ull;
-        const bool sp = lds_row<KB, NV>(tab, true, 0, lane, key, spec, kraw, cv, valid, probe_limit, peel_min);
-        if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kraw, cv, peel_min, ins);
+        for (int j = 0; j < J; ++j) {
+            uint64_t cv[NV];
+            canon_row<VL, NV>(spec, raw[j], cv);
+            const bool sp = lds_row<KB, NV>(tab, true, 0, lane, key, spec, kr[j], cv, valid[j], probe_limit, peel_min);
+            if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kr[j], cv, peel_min, ins);
+        }
     }
     __syncthreads();
     // finished groups of this unit
@@ -162,20 +198,16 @@ int ceil_log2(int64_t x)
 template <int KB>
 void launch_part_agg(const KeyDesc& key, const AggSpec& spec, const int64_t* perm, const Unit* units, int nunits,
                      const GTable& gt, const OutSpec& os, unsigned long long* counter, int64_t capacity, int cr_log2,
-                     size_t lds, hipStream_t s)
+                     size_t lds, int block, hipStream_t s)
 {
-    const int nv = spec.nvcols;
-    auto go = [&](auto kern) {
+    with_vlayout(spec, true, [&](auto vl) {
+        auto kern = part_agg_kernel<KB, decltype(vl)>;
         if (enable_dynamic_lds((const void*)kern) < lds) raise(EG_ERR_UNSUPPORTED, "partition LDS table too large");
         KernelTimer timer("partition_agg", s);
-        hipLaunchKernelGGL(kern, dim3(nunits), dim3(kPB), lds, s, key, spec, perm, units, gt, os, counter, capacity,
+        hipLaunchKernelGGL(kern, dim3(nunits), dim3(block), lds, s, key, spec, perm, units, gt, os, counter, capacity,
                            cr_log2, 64, 8);
         EG_LAUNCH_CHECK();
-    };
-    if (nv <= 1) go(part_agg_kernel<KB, 1>);
-    else if (nv <= 2) go(part_agg_kernel<KB, 2>);
-    else if (nv <= 4) go(part_agg_kernel<KB, 4>);
-    else go(part_agg_kernel<KB, 8>);
+    });
 }
 
 }  // namespace
@@ -188,8 +220,24 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     const int nops = plan.spec.nops;
     const int64_t bound = std::max<int64_t>(1, std::min<int64_t>(group_bound, n));
 
-    // LDS table: the largest power of two <= 2048 slots that fits one block
-    int cr_log2 = 11;
+    // LDS table and workgroup: 1024 slots with 512 threads runs the unit
+    // kernel about 1.5x faster than 2048 slots with 1024 threads (three
+    // resident workgroups per CU instead of one), but doubles the partition
+    // count; take it when that costs no extra radix pass.
+    PartTune tune = part_tune();
+    auto bits_for = [&](int c) {
+        const int64_t pp = (int64_t)1 << (c - 1);
+        return std::min(24, std::max(1, ceil_log2((bound + pp - 1) / pp)));
+    };
+    if (!tune.fixed) {
+        const int p11 = (bits_for(11) + kMaxDigitBits - 1) / kMaxDigitBits;
+        const int p10 = (bits_for(10) + kMaxDigitBits - 1) / kMaxDigitBits;
+        if (p10 == p11 && bound > ((int64_t)1 << 12)) {
+            tune.cr_log2 = 10;
+            tune.block = 512;
+        }
+    }
+    int cr_log2 = tune.cr_log2;
     auto lds_of = [&](int c) {
         const size_t T = (size_t)(1 << c) + 1;
         return (size_t)nops * T * 8 + (T * kb + 7) / 8 * 8 + 8;
@@ -200,6 +248,7 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     const int bits = std::min(24, std::max(1, ceil_log2((bound + per_part - 1) / per_part)));
     const uint32_t pmask = (uint32_t)((1u << bits) - 1u);
 
+    std::unique_ptr<HostPhase> phase_prep(new HostPhase("host_partition_prep", s));
     // ---- radix passes on the hash bits -----------------------------------------
     RadixKeys rk{};
     rk.data = plan.key.data;
@@ -229,13 +278,14 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
             b.pay_b.push_back(bufs[bufs.size() - 1].get());
         }
     }
-    // balanced digits: ceil(bits / 8) passes of near-equal width
-    const int passes = (bits + 7) / 8;
+    // balanced digits: ceil(bits / kMaxDigitBits) passes of near-equal width
+    phase_prep.reset();
+    const int passes = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
     const void* pkeys = nullptr;
     std::vector<const void*> ppay;
     {
-        // radix_sort walks [0, bits) in 8-bit steps; balance by sorting the
-        // low passes first with equal widths
+        HostPhase phase("host_partition_radix", s);
+        // LSD passes over [0, bits) of near-equal width
         const int w = (bits + passes - 1) / passes;
         const void* kin = b.keys_in;
         std::vector<Payload> pay = b.pay_in;
@@ -269,6 +319,7 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
 
     // ---- partition boundaries and units -------------------------------------------
     const int64_t P = (int64_t)pmask + 1;
+    std::unique_ptr<HostPhase> phase_units(new HostPhase("host_partition_units", s));
     DevBuf offs((size_t)(P + 1) * 8, s);
     {
         const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)di.num_cus * 8));
@@ -286,7 +337,7 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     // atomics instead of being written straight out.
     const int64_t avg_rows = (n + P - 1) / P;
     const int64_t balance = std::max<int64_t>(1, n / (4 * (int64_t)di.num_cus));
-    const int64_t slice = std::max<int64_t>({(int64_t)kPB * 16, 8 * per_part, std::min(4 * avg_rows, balance)});
+    const int64_t slice = std::max<int64_t>({(int64_t)tune.block * 16, 8 * per_part, std::min(4 * avg_rows, balance)});
     std::vector<Unit> units;
     for (int64_t p = 0; p < P; ++p) {
         const int64_t a = ho[(size_t)p], e = ho[(size_t)p + 1];
@@ -301,6 +352,8 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     DevBuf dunits(units.size() * sizeof(Unit), s);
     EG_HIP(hipMemcpyAsync(dunits.get(), units.data(), units.size() * sizeof(Unit), hipMemcpyHostToDevice, s));
 
+    phase_units.reset();
+    HostPhase phase_agg("host_partition_aggregate", s);
     // ---- aggregation ------------------------------------------------------------------
     // HBM table: groups of sliced partitions plus LDS spills
     int64_t sliced_parts = 0;
@@ -321,10 +374,10 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
         fill_zero(counter.get(), 8, s);
         if (kb == 4)
             launch_part_agg<4>(pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
-                               counter.get<unsigned long long>(), capacity, cr_log2, lds, s);
+                               counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, s);
         else
             launch_part_agg<8>(pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
-                               counter.get<unsigned long long>(), capacity, cr_log2, lds, s);
+                               counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, s);
         int64_t meta[3];
         ht.read_meta(meta, s);
         const bool table_full = meta[2] != 0;

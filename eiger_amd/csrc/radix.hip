@@ -27,6 +27,10 @@ namespace eg {
 namespace {
 
 constexpr int kRB = 256;        // threads per block
+// Digit width of a sort pass: fan-out 64 keeps each digit's rows in runs long
+// enough for full-line stores (a 6-bit pass on 2^28 rows takes about half the
+// time of an 8-bit one), so a 32-bit key takes six passes instead of four.
+constexpr int kSortDigitBits = 6;
 constexpr int kRI = 16;         // rows per thread
 constexpr int kRT = kRB * kRI;  // rows per tile
 
@@ -47,6 +51,27 @@ __device__ __forceinline__ uint32_t digit_of(const RadixKeys& k, uint64_t raw, i
     return k.desc ? mask - d : d;
 }
 
+template <int KB>
+__device__ __forceinline__ uint64_t load_key(const void* p, int64_t i)
+{
+    if constexpr (KB == 1) return ((const uint8_t*)p)[i];
+    else if constexpr (KB == 2) return ((const uint16_t*)p)[i];
+    else if constexpr (KB == 4) return ((const uint32_t*)p)[i];
+    else return ((const uint64_t*)p)[i];
+}
+
+template <int KB>
+__device__ __forceinline__ void store_key(void* p, int64_t i, uint64_t v)
+{
+    if constexpr (KB == 1) ((uint8_t*)p)[i] = (uint8_t)v;
+    else if constexpr (KB == 2) ((uint16_t*)p)[i] = (uint16_t)v;
+    else if constexpr (KB == 4) ((uint32_t*)p)[i] = (uint32_t)v;
+    else ((uint64_t*)p)[i] = v;
+}
+
+// Key widths are template parameters so the unrolled loads of a thread are
+// independent (a runtime width switch around each load serialises them).
+template <int KB>
 __global__ __launch_bounds__(kRB) void radix_hist_kernel(RadixKeys k, int64_t n, int shift, uint32_t mask,
                                                           uint32_t* __restrict__ hist, int ntiles)
 {
@@ -55,10 +80,16 @@ __global__ __launch_bounds__(kRB) void radix_hist_kernel(RadixKeys k, int64_t n,
     h[t] = 0u;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kRT;
+    uint64_t raw[kRI];
 #pragma unroll
     for (int i = 0; i < kRI; ++i) {
         const int64_t e = base + (int64_t)i * kRB + t;
-        if (e < n) atomicAdd(&h[digit_of(k, dev::load_raw(k.data, k.bytes, e), shift, mask)], 1u);
+        raw[i] = e < n ? load_key<KB>(k.data, e) : 0ull;
+    }
+#pragma unroll
+    for (int i = 0; i < kRI; ++i) {
+        const int64_t e = base + (int64_t)i * kRB + t;
+        if (e < n) atomicAdd(&h[digit_of(k, raw[i], shift, mask)], 1u);
     }
     __syncthreads();
     hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
@@ -82,12 +113,44 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wsu
     return x - v + add;
 }
 
+// Writes one column of the tile in digit order: values go to their tile
+// position in LDS, then leave in runs of one digit (coalesced stores).
+// `vals` holds the row's value when `have` (the keys, already in registers).
+template <int B>
+__device__ __forceinline__ void scatter_column(const void* src, void* dst, bool rid, bool have,
+                                               const uint64_t (&vals)[kRI], const uint32_t (&pos)[kRI],
+                                               int64_t wbase, int lane, int t, int32_t tcount,
+                                               unsigned long long* stage, const uint8_t* sdig,
+                                               const uint32_t* gofs, const uint32_t* lstart)
+{
+    __syncthreads();
+    uint64_t v[kRI];
+#pragma unroll
+    for (int i = 0; i < kRI; ++i) {
+        const int64_t e = wbase + (int64_t)i * 64 + lane;
+        v[i] = have ? vals[i] : (rid ? (uint64_t)e : (pos[i] != 0xffffffffu ? load_key<B>(src, e) : 0ull));
+    }
+#pragma unroll
+    for (int i = 0; i < kRI; ++i)
+        if (pos[i] != 0xffffffffu) stage[pos[i]] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRI; ++i) {
+        const int p = i * kRB + t;
+        if (p < tcount) {
+            const uint32_t d = sdig[p];
+            store_key<B>(dst, (int64_t)gofs[d] + (p - (int64_t)lstart[d]), stage[p]);
+        }
+    }
+}
+
 // Stable multisplit of one 4096-row tile.  Wave w owns rows
 // [w*1024, (w+1)*1024) of the tile and walks them 64 at a time: the lanes
 // with the same digit are found with one ballot per digit bit, the lowest of
 // them advances the wave's running count of that digit in LDS.  No barrier is
 // needed inside the walk (each wave updates only its own counters); one scan
 // over (wave, digit) then gives every row its place in the tile.
+template <int KB>
 __global__ __launch_bounds__(kRB) void radix_scatter_kernel(RadixKeys k, void* __restrict__ kout, PayArgs pay,
                                                              int64_t n, int shift, int bits,
                                                              const uint32_t* __restrict__ offs, int ntiles)
@@ -108,16 +171,23 @@ __global__ __launch_bounds__(kRB) void radix_scatter_kernel(RadixKeys k, void* _
 #pragma unroll
     for (int w = 0; w < kW; ++w) whist[w][t] = 0u;
     gofs[t] = offs[(int64_t)t * ntiles + blockIdx.x];
+
+    const int64_t wbase = tbase + (int64_t)wave * kWR;
+    uint64_t kr[kRI];
+#pragma unroll
+    for (int i = 0; i < kRI; ++i) {
+        const int64_t e = wbase + (int64_t)i * 64 + lane;
+        kr[i] = e < n ? load_key<KB>(k.data, e) : 0ull;
+    }
     __syncthreads();
 
     uint32_t dg[kRI], rk[kRI];
     const uint64_t lt = dev::lanemask_lt();
-    const int64_t wbase = tbase + (int64_t)wave * kWR;
 #pragma unroll
     for (int i = 0; i < kRI; ++i) {
         const int64_t e = wbase + (int64_t)i * 64 + lane;
         const bool valid = e < n;
-        const uint32_t d = valid ? digit_of(k, dev::load_raw(k.data, k.bytes, e), shift, mask) : 0u;
+        const uint32_t d = valid ? digit_of(k, kr[i], shift, mask) : 0u;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < bits; ++b) {
             const bool bit = (d >> b) & 1u;
@@ -159,27 +229,23 @@ __global__ __launch_bounds__(kRB) void radix_scatter_kernel(RadixKeys k, void* _
         }
     }
 
-    // keys, then every payload column, through the LDS stage
-    for (int c = -1; c < pay.n; ++c) {
-        const void* src = c < 0 ? k.data : pay.in[c];
-        void* dst = c < 0 ? kout : pay.out[c];
-        const int bytes = c < 0 ? k.bytes : pay.bytes[c];
-        const bool rid = c >= 0 && pay.rowid[c] != 0;
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < kRI; ++i)
-            if (pos[i] != 0xffffffffu) {
-                const int64_t e = wbase + (int64_t)i * 64 + lane;
-                stage[pos[i]] = rid ? (unsigned long long)e : (unsigned long long)dev::load_raw(src, bytes, e);
-            }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < kRI; ++i) {
-            const int p = i * kRB + t;
-            if (p < tcount) {
-                const uint32_t d = sdig[p];
-                dev::store_bytes(dst, bytes, (int64_t)gofs[d] + (p - (int64_t)lstart[d]), stage[p]);
-            }
+    // keys (from registers), then every payload column, through the LDS stage
+    scatter_column<KB>(nullptr, kout, false, true, kr, pos, wbase, lane, t, tcount, stage, sdig, gofs, lstart);
+    for (int c = 0; c < pay.n; ++c) {
+        const bool rid = pay.rowid[c] != 0;
+        switch (rid ? 8 : pay.bytes[c]) {
+        case 1:
+            scatter_column<1>(pay.in[c], pay.out[c], false, false, kr, pos, wbase, lane, t, tcount, stage, sdig, gofs, lstart);
+            break;
+        case 2:
+            scatter_column<2>(pay.in[c], pay.out[c], false, false, kr, pos, wbase, lane, t, tcount, stage, sdig, gofs, lstart);
+            break;
+        case 4:
+            scatter_column<4>(pay.in[c], pay.out[c], false, false, kr, pos, wbase, lane, t, tcount, stage, sdig, gofs, lstart);
+            break;
+        default:
+            scatter_column<8>(pay.in[c], pay.out[c], rid, false, kr, pos, wbase, lane, t, tcount, stage, sdig, gofs, lstart);
+            break;
         }
     }
 }
@@ -328,8 +394,9 @@ void radix_pass(const RadixKeys& k, void* keys_out, const std::vector<Payload>& 
     DevBuf hist((size_t)256 * ntiles * 4, s), offs((size_t)256 * ntiles * 4, s);
     {
         KernelTimer timer("radix_hist", s);
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(ntiles), dim3(kRB), 0, s, k, n, shift, mask, hist.get<uint32_t>(),
-                           ntiles);
+        auto hk = k.bytes == 1 ? radix_hist_kernel<1> : k.bytes == 2 ? radix_hist_kernel<2>
+                : k.bytes == 4 ? radix_hist_kernel<4> : radix_hist_kernel<8>;
+        hipLaunchKernelGGL(hk, dim3(ntiles), dim3(kRB), 0, s, k, n, shift, mask, hist.get<uint32_t>(), ntiles);
         EG_LAUNCH_CHECK();
     }
     {
@@ -345,7 +412,9 @@ void radix_pass(const RadixKeys& k, void* keys_out, const std::vector<Payload>& 
         pa.rowid[c] = pay[c].rowid;
     }
     KernelTimer timer("radix_scatter", s);
-    hipLaunchKernelGGL(radix_scatter_kernel, dim3(ntiles), dim3(kRB), 0, s, k, keys_out, pa, n, shift, bits,
+    auto sk = k.bytes == 1 ? radix_scatter_kernel<1> : k.bytes == 2 ? radix_scatter_kernel<2>
+            : k.bytes == 4 ? radix_scatter_kernel<4> : radix_scatter_kernel<8>;
+    hipLaunchKernelGGL(sk, dim3(ntiles), dim3(kRB), 0, s, k, keys_out, pa, n, shift, bits,
                        (const uint32_t*)offs.get<uint32_t>(), ntiles);
     EG_LAUNCH_CHECK();
 }
@@ -381,8 +450,12 @@ void radix_sort(const RadixKeys& k0, SortBuffers& b, int64_t n, int begin_bit, i
     std::vector<Payload> pay(np);
     for (size_t c = 0; c < np; ++c) pay[c] = b.pay_in[c];
     bool to_a = true;
-    for (int shift = begin_bit; shift < end_bit; shift += 8) {
-        const int bits = std::min(8, end_bit - shift);
+    // balanced passes of at most kSortDigitBits bits
+    const int total = end_bit - begin_bit;
+    const int passes = (total + kSortDigitBits - 1) / kSortDigitBits;
+    const int width = (total + passes - 1) / passes;
+    for (int shift = begin_bit; shift < end_bit; shift += width) {
+        const int bits = std::min(width, end_bit - shift);
         void* kout = to_a ? b.keys_a : b.keys_b;
         for (size_t c = 0; c < np; ++c) pay[c].out = to_a ? b.pay_a[c] : b.pay_b[c];
         k.data = kin;

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <chrono>
 #include <mutex>
 #include <unordered_map>
 
@@ -121,13 +122,43 @@ struct Pool {
 };
 
 std::mutex g_mm_mu;
-int g_mode = EG_MM_ASYNC;
+int g_mode = EG_MM_CACHE;
 std::unordered_map<void*, Alloc> g_allocs;
 int64_t g_cur = 0, g_peak = 0;
 Pool g_pool;
 bool g_pool_threshold_set = false;
 
 constexpr size_t kAlign = 256;
+
+// Size classes of the caching allocator: four per power of two (<= 25% slack).
+size_t cache_class(size_t bytes)
+{
+    if (bytes <= 4096) return 4096;
+    int lg = 63 - __builtin_clzll((unsigned long long)bytes);
+    const size_t step = (size_t)1 << (lg - 2);
+    return (bytes + step - 1) / step * step;
+}
+
+struct CacheKey {
+    hipStream_t stream;
+    size_t bytes;
+    bool operator<(const CacheKey& o) const
+    {
+        return stream != o.stream ? std::less<hipStream_t>()(stream, o.stream) : bytes < o.bytes;
+    }
+};
+std::map<CacheKey, std::vector<void*>> g_cache;
+int64_t g_cached = 0;
+
+void cache_release_all()
+{
+    // every cached block may still be referenced by queued work on its stream
+    (void)hipDeviceSynchronize();
+    for (auto& kv : g_cache)
+        for (void* p : kv.second) (void)hipFree(p);
+    g_cache.clear();
+    g_cached = 0;
+}
 
 void* pool_alloc(size_t bytes)
 {
@@ -171,6 +202,26 @@ void* mm_alloc(size_t bytes, hipStream_t s)
     if (g_mode == EG_MM_POOL) {
         p = pool_alloc(bytes);
         if (!p) throw std::bad_alloc();
+    } else if (g_mode == EG_MM_CACHE) {
+        bytes = cache_class(bytes);
+        auto it = g_cache.find(CacheKey{s, bytes});
+        if (it != g_cache.end() && !it->second.empty()) {
+            p = it->second.back();
+            it->second.pop_back();
+            g_cached -= (int64_t)bytes;
+        } else {
+            hipError_t e = hipMalloc(&p, bytes);
+            if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+                (void)hipGetLastError();
+                cache_release_all();
+                e = hipMalloc(&p, bytes);
+            }
+            if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+                (void)hipGetLastError();
+                throw std::bad_alloc();
+            }
+            if (e != hipSuccess) raise(EG_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
     } else {
         if (!g_pool_threshold_set) {
             int dev = 0;
@@ -204,7 +255,10 @@ void mm_free(void* p)
     g_allocs.erase(it);
     g_cur -= (int64_t)a.bytes;
     if (g_mode == EG_MM_POOL) pool_free(p, a.bytes);
-    else EG_HIP(hipFreeAsync(p, a.stream));
+    else if (g_mode == EG_MM_CACHE) {
+        g_cache[CacheKey{a.stream, a.bytes}].push_back(p);
+        g_cached += (int64_t)a.bytes;
+    } else EG_HIP(hipFreeAsync(p, a.stream));
 }
 
 void mm_usage(int64_t* cur, int64_t* peak)
@@ -218,7 +272,8 @@ void mm_set_mode(int mode, int64_t pool_bytes)
 {
     std::lock_guard<std::mutex> lk(g_mm_mu);
     if (!g_allocs.empty()) raise(EG_ERR_INVALID, "memory mode can only change while nothing is allocated");
-    if (mode != EG_MM_ASYNC && mode != EG_MM_POOL) raise(EG_ERR_INVALID, "unknown memory mode");
+    if (mode != EG_MM_ASYNC && mode != EG_MM_POOL && mode != EG_MM_CACHE) raise(EG_ERR_INVALID, "unknown memory mode");
+    cache_release_all();
     if (g_pool.base) {
         EG_HIP(hipFree(g_pool.base));
         g_pool = Pool{};
@@ -259,6 +314,7 @@ namespace {
 struct TimingRec {
     std::string name;
     hipEvent_t a, b;
+    double host_ms;  // host phases (a == nullptr)
 };
 std::mutex g_t_mu;
 bool g_timing = false;
@@ -280,7 +336,24 @@ KernelTimer::~KernelTimer()
     if (!a_) return;
     if (hipEventRecord(b_, s_) != hipSuccess) return;
     std::lock_guard<std::mutex> lk(g_t_mu);
-    g_recs.push_back(TimingRec{name_, a_, b_});
+    g_recs.push_back(TimingRec{name_, a_, b_, 0.0});
+}
+
+HostPhase::HostPhase(const char* name, hipStream_t s) : name_(name), s_(s)
+{
+    if (!g_timing) return;
+    on_ = true;
+    (void)hipStreamSynchronize(s_);
+    t0_ = std::chrono::steady_clock::now();
+}
+
+HostPhase::~HostPhase()
+{
+    if (!on_) return;
+    (void)hipStreamSynchronize(s_);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    g_recs.push_back(TimingRec{name_, nullptr, nullptr, ms});
 }
 
 int timing_enable(int on)
@@ -293,6 +366,7 @@ int timing_reset()
 {
     std::lock_guard<std::mutex> lk(g_t_mu);
     for (auto& r : g_recs) {
+        if (!r.a) continue;  // host phase: no events
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
@@ -307,6 +381,11 @@ void timing_query(const char* name, int64_t* launches, double* total_ms)
     double tot = 0.0;
     for (auto& r : g_recs) {
         if (r.name != name) continue;
+        if (!r.a) {
+            tot += r.host_ms;
+            ++n;
+            continue;
+        }
         EG_HIP(hipEventSynchronize(r.b));
         float ms = 0.f;
         EG_HIP(hipEventElapsedTime(&ms, r.a, r.b));

@@ -26,10 +26,13 @@
 
 #include "eg_device.hpp"
 #include "eg_internal.hpp"
+#include "eg_lds_table.hpp"
 #include "eg_radix.hpp"
 
 namespace eg {
 namespace {
+
+using namespace lds;
 
 constexpr int kSegB = 256;
 
@@ -94,79 +97,181 @@ __device__ __forceinline__ uint32_t owner_tile(const uint32_t* head_off, uint32_
     return lo;
 }
 
-// States of local group lg, op a at st[a * T + lg].
-template <int ITEMS, int NV>
-__global__ __launch_bounds__(kSegB) void seg_reduce_kernel(const void* keys, int kb, int is_signed, AggSpec spec,
+// Writes group g from states held in registers.
+__device__ __forceinline__ void emit_regs(const OutSpec& os, int64_t g, uint64_t key_enc, const uint64_t (&acc)[kMaxOps])
+{
+    dev::store_bytes(os.key_data, os.key_bytes, g, dev::dec64(key_enc));
+    for (int j = 0; j < os.nouts; ++j) {
+        const OutDesc& o = os.outs[j];
+        uint64_t st[2] = {0ull, 0ull};
+#pragma unroll
+        for (int a = 0; a < kMaxOps; ++a) {
+            if (a == o.op) st[0] = acc[a];
+            if (a == o.op2) st[1] = acc[a];
+        }
+        const uint64_t s = st[0];
+        switch (o.code) {
+        case OUT_U64: dev::store_bytes(o.data, 8, g, s); break;
+        case OUT_F32: ((float*)o.data)[g] = (float)__longlong_as_double((long long)s); break;
+        case OUT_F64: ((double*)o.data)[g] = __longlong_as_double((long long)s); break;
+        case OUT_MAX_I: dev::store_bytes(o.data, o.bytes, g, s ^ dev::kSign64); break;
+        case OUT_MIN_I: dev::store_bytes(o.data, o.bytes, g, (~s) ^ dev::kSign64); break;
+        case OUT_MAX_U: dev::store_bytes(o.data, o.bytes, g, s); break;
+        case OUT_MIN_U: dev::store_bytes(o.data, o.bytes, g, ~s); break;
+        case OUT_MAX_F: dev::store_double_as(o.data, o.bytes, g, dev::unord_f64(s)); break;
+        case OUT_MIN_F: dev::store_double_as(o.data, o.bytes, g, dev::unord_f64(~s)); break;
+        case OUT_ANY:
+            if (o.is_float && o.bytes == 4) ((float*)o.data)[g] = (float)__longlong_as_double((long long)s);
+            else dev::store_bytes(o.data, o.bytes, g, s);
+            break;
+        default: {
+            const double cnt = (double)st[1];
+            const double sum = (o.code == OUT_AVG_I) ? (double)(int64_t)s
+                             : (o.code == OUT_AVG_U) ? (double)s
+                             : __longlong_as_double((long long)s);
+            ((double*)o.data)[g] = sum / cnt;
+            break;
+        }
+        }
+    }
+}
+
+// One 4096-row tile of sorted rows (ITEMS consecutive rows per lane).  A run
+// (rows of one key) that starts and ends inside a lane is folded in registers
+// and written straight out.  Runs that cross lane boundaries meet in LDS:
+// slot L (nops states) collects the group whose head is in lane L, slot
+// kSegB the group whose head lies before the tile.  A continuation run finds
+// its slot through a max-scan of "last lane with a head".  The tile's last
+// group, if it continues into the next tile, goes to the tile's boundary
+// state instead; the continuation group of an earlier tile's head is added
+// to that tile's boundary state.
+template <int ITEMS, int KB, class VL>
+__global__ __launch_bounds__(kSegB) void seg_reduce_kernel(const void* keys, int kb_unused, int is_signed, AggSpec spec,
                                                            const int64_t* perm, int64_t n,
                                                            const uint32_t* __restrict__ head_off, uint32_t ntiles,
                                                            OutSpec os, unsigned long long* bstate,
                                                            unsigned long long* bkey, uint32_t* btrail)
 {
     constexpr int T = kSegB * ITEMS;
-    extern __shared__ unsigned long long smem[];
-    unsigned long long* st = smem;                                  // nops * T
-    unsigned long long* gkey = smem + (size_t)spec.nops * T;       // T
+    constexpr int kSlots = kSegB + 1;
+    constexpr int NV = VL::NV > 0 ? VL::NV : 1;
+    constexpr int kb = KB;
+    (void)kb_unused;
+    extern __shared__ unsigned long long slot[];  // nops * kSlots
     __shared__ uint32_t wsum[kSegB / 64];
-    __shared__ uint32_t s_first_cont, s_last_cont;
+    __shared__ int wmax[kSegB / 64];
+    __shared__ uint32_t s_tile_cont;
 
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint32_t tile = blockIdx.x;
     const int64_t tstart = (int64_t)tile * T;
     const int64_t r0 = tstart + (int64_t)t * ITEMS;
     const int nops = spec.nops;
+    for (int i = t; i < nops * kSlots; i += kSegB) slot[i] = 0ull;
+    if (t == 0) s_tile_cont = 0u;
 
     uint64_t k[ITEMS];
-    bool real_head[ITEMS];
-    uint32_t nforced = 0;
-    uint64_t prev = (r0 > 0 && r0 < n) ? dev::load_raw(keys, kb, r0 - 1) : 0ull;
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const int64_t r = r0 + i;
-        k[i] = r < n ? dev::load_raw(keys, kb, r) : 0ull;
-        real_head[i] = r < n && (r == 0 || k[i] != prev);
-        const bool forced = r < n && (real_head[i] || r == tstart);
-        nforced += forced ? 1u : 0u;
-        prev = k[i];
-    }
-    if (t == 0) s_first_cont = (tstart > 0 && !real_head[0]) ? 1u : 0u;
-    if (t == kSegB - 1) {
-        const int64_t nx = tstart + T;
-        s_last_cont = (nx < n && dev::load_raw(keys, kb, nx) == dev::load_raw(keys, kb, nx - 1)) ? 1u : 0u;
-    }
-    uint32_t G = 0;
-    const uint32_t lg0 = block_excl_scan(nforced, wsum, &G);
-    for (int i = t; i < nops * (int)G; i += kSegB) {
-        const int a = i / (int)G, g = i - a * (int)G;
-        st[(size_t)a * T + g] = 0ull;
-    }
-    __syncthreads();
-
-    // fold this lane's rows; flush each run into its local group's LDS states
-    uint64_t acc[kMaxOps];
-    int lg = (int)lg0 - 1;
-    bool have = false;
-    auto flush = [&]() {
-#pragma unroll
-        for (int a = 0; a < kMaxOps; ++a)
-            if (a < nops) dev::apply_atomic(dev::class_of(spec, a), &st[(size_t)a * T + lg], acc[a]);
+    bool head[ITEMS];
+    uint32_t nheads = 0;
+    auto key_at = [&](int64_t r) -> uint64_t {
+        return KB == 4 ? (uint64_t)((const uint32_t*)keys)[r] : ((const uint64_t*)keys)[r];
     };
 #pragma unroll
+    for (int i = 0; i < ITEMS; ++i) k[i] = r0 + i < n ? key_at(r0 + i) : 0ull;
+    uint64_t prev = (r0 > 0 && r0 < n) ? key_at(r0 - 1) : 0ull;
+#pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const int64_t r = r0 + i;
-        if (r >= n) break;
-        const bool forced = real_head[i] || r == tstart;
-        if (forced) {
-            if (have) flush();
-            ++lg;
-            if (real_head[i] || r == tstart) gkey[lg] = k[i];
-        }
-        const int64_t src = perm ? perm[r] : r;
-        uint64_t cv[NV];
+        head[i] = r < n && (r == 0 || k[i] != prev);
+        nheads += head[i] ? 1u : 0u;
+        prev = k[i];
+    }
+    const int64_t r_end = r0 + ITEMS;  // first row after this lane
+    const bool next_head = r_end >= n || (r0 < n && key_at(r_end) != k[ITEMS - 1]);
+    // groups whose head precedes this lane within the tile, and the tile total
+    uint32_t tile_heads = 0;
+    const uint32_t hbefore = block_excl_scan(nheads, wsum, &tile_heads);
+    // owner of a continuation: the last lane before this one with a head (-1: before the tile)
+    int mine = nheads > 0 ? t : -1;
+    int incl = mine;
 #pragma unroll
-        for (int c = 0; c < NV; ++c)
-            cv[c] = c < spec.nvcols ? dev::canon(dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, src),
-                                                 spec.cols[c].conv)
-                                    : 0ull;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off);
+        if (lane >= off) incl = y > incl ? y : incl;
+    }
+    if (lane == 63) wmax[w] = incl;
+    __syncthreads();  // also orders the slot clearing above
+    int prev_max = -1;
+    for (int i = 0; i < w; ++i) prev_max = wmax[i] > prev_max ? wmax[i] : prev_max;
+    int excl = __shfl_up(incl, 1);
+    if (lane == 0) excl = -1;
+    excl = excl > prev_max ? excl : prev_max;
+    const int owner = excl < 0 ? kSegB : excl;  // slot of a continuation run
+
+    const uint32_t hbase = head_off[tile];
+    const bool last_cont_tile = [&] {
+        const int64_t nx = tstart + T;
+        return nx < n && key_at(nx) == key_at(nx - 1);
+    }();
+
+    uint64_t acc[kMaxOps];
+#pragma unroll
+    for (int a = 0; a < kMaxOps; ++a) acc[a] = 0ull;
+    bool have = false, cont = r0 < n && r0 > 0 && !head[0];
+    uint32_t hseen = 0;          // heads of this lane seen so far
+    uint64_t run_key = 0;
+    auto to_slot = [&](int sidx) {
+#pragma unroll
+        for (int a = 0; a < kMaxOps; ++a)
+            if (a < nops) dev::apply_atomic(dev::class_of(spec, a), &slot[(size_t)a * kSlots + sidx], acc[a]);
+    };
+    // closes the current run; `closed`: the run ends inside this lane
+    auto finish = [&](bool closed) {
+        if (!have) return;
+        if (cont) {
+            to_slot(owner);
+            if (owner == kSegB) s_tile_cont = 1u;
+        } else if (closed) {
+            emit_regs(os, (int64_t)hbase + hbefore + hseen - 1, dev::enc64(dev::widen_key(run_key, kb, is_signed)), acc);
+        } else {
+            to_slot(t);
+        }
+    };
+    // values of the lane's rows: typed loads issued eight rows at a time
+    // (fixed layouts), then folded
+    constexpr int H = 8;
+    static_assert(ITEMS % H == 0, "rows per lane in halves");
+#pragma unroll
+    for (int h0 = 0; h0 < ITEMS; h0 += H) {
+    uint64_t rawv[VL::GEN ? 1 : H][NV];
+    if constexpr (!VL::GEN) {
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const int64_t r = r0 + h0 + j;
+            const bool valid = r < n;
+            load_row<VL>(spec, valid ? (perm ? perm[r] : r) : 0, valid, rawv[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        const int i = h0 + j;
+        const int64_t r = r0 + i;
+        if (r >= n) break;
+        if (head[i]) {
+            finish(true);
+            have = false;
+            cont = false;
+            ++hseen;
+            run_key = k[i];
+        }
+        uint64_t cv[NV];
+        if constexpr (VL::GEN) {
+            uint64_t raw[NV];
+            load_row<VL>(spec, perm ? perm[r] : r, true, raw);
+            canon_row<VL, NV>(spec, raw, cv);
+        } else {
+            canon_row<VL, NV>(spec, rawv[j], cv);
+        }
 #pragma unroll
         for (int a = 0; a < kMaxOps; ++a) {
             if (a < nops) {
@@ -176,32 +281,38 @@ __global__ __launch_bounds__(kSegB) void seg_reduce_kernel(const void* keys, int
                 for (int c = 0; c < NV; ++c)
                     if (c == od.col) v = cv[c];
                 const uint64_t o = dev::operand(od, v);
-                acc[a] = (forced || !have) ? o : dev::combine(dev::class_of(spec, a), acc[a], o);
+                acc[a] = have ? dev::combine(dev::class_of(spec, a), acc[a], o) : o;
             }
         }
         have = true;
     }
-    if (have) flush();
+    }
+    // the lane's last run: complete when the next row starts a new group
+    finish(next_head);
     __syncthreads();
 
-    const bool first_cont = s_first_cont != 0u;
-    const bool last_cont = s_last_cont != 0u;
-    const uint32_t hbase = head_off[tile];
-    for (int g = t; g < (int)G; g += kSegB) {
-        const uint32_t gidx = hbase + (uint32_t)g - (first_cont ? 1u : 0u);
-        if (first_cont && g == 0) {
-            // continuation of a group whose head is in an earlier tile
-            const uint32_t owner = owner_tile(head_off, ntiles, gidx);
-            unsigned long long* dst = bstate + (size_t)owner * nops;
-            for (int a = 0; a < nops; ++a) dev::apply_atomic(dev::class_of(spec, a), dst + a, st[(size_t)a * T + g]);
-        } else if (last_cont && g == (int)G - 1) {
+    // groups collected in slots: the one with its head in this lane
+    const bool owns_open = have && !cont && !next_head;
+    if (owns_open) {
+        const uint32_t gidx = hbase + hbefore + hseen - 1;
+        if (last_cont_tile && hbefore + hseen == tile_heads) {
+            // the tile's last group continues into the next tile
             unsigned long long* dst = bstate + (size_t)tile * nops;
-            for (int a = 0; a < nops; ++a) dev::apply_atomic(dev::class_of(spec, a), dst + a, st[(size_t)a * T + g]);
-            bkey[tile] = gkey[g];
+            for (int a = 0; a < nops; ++a)
+                dev::apply_atomic(dev::class_of(spec, a), dst + a, slot[(size_t)a * kSlots + t]);
+            bkey[tile] = run_key;
             btrail[tile] = 1u;
         } else {
-            dev::emit_group(os, (int64_t)gidx, dev::enc64(dev::widen_key(gkey[g], kb, is_signed)), st + g, (int64_t)T);
+            dev::emit_group(os, (int64_t)gidx, dev::enc64(dev::widen_key(run_key, kb, is_signed)), slot + t,
+                            (int64_t)kSlots);
         }
+    }
+    if (t == 0 && s_tile_cont) {
+        // continuation of a group whose head is in an earlier tile
+        const uint32_t own = owner_tile(head_off, ntiles, hbase - 1u);
+        unsigned long long* dst = bstate + (size_t)own * nops;
+        for (int a = 0; a < nops; ++a)
+            dev::apply_atomic(dev::class_of(spec, a), dst + a, slot[(size_t)a * kSlots + kSegB]);
     }
 }
 
@@ -216,31 +327,24 @@ __global__ void seg_tail_kernel(uint32_t ntiles, const uint32_t* head_off, const
                     (int64_t)1);
 }
 
-template <int ITEMS, int NV>
+template <int ITEMS>
 void launch_seg(const void* keys, int kb, int is_signed, const AggSpec& spec, const int64_t* perm, int64_t n,
                 const uint32_t* head_off, uint32_t ntiles, const OutSpec& os, unsigned long long* bstate,
                 unsigned long long* bkey, uint32_t* btrail, hipStream_t s)
 {
-    constexpr int T = kSegB * ITEMS;
-    const size_t lds = (size_t)(spec.nops + 1) * T * 8;
-    auto kern = seg_reduce_kernel<ITEMS, NV>;
-    if (enable_dynamic_lds((const void*)kern) < lds) raise(EG_ERR_UNSUPPORTED, "segmented reduction tile too large");
-    KernelTimer timer("segreduce", s);
-    hipLaunchKernelGGL(kern, dim3(ntiles), dim3(kSegB), lds, s, keys, kb, is_signed, spec, perm, n, head_off, ntiles,
-                       os, bstate, bkey, btrail);
-    EG_LAUNCH_CHECK();
-}
-
-template <int ITEMS>
-void launch_seg_nv(const void* keys, int kb, int is_signed, const AggSpec& spec, const int64_t* perm, int64_t n,
-                   const uint32_t* head_off, uint32_t ntiles, const OutSpec& os, unsigned long long* bstate,
-                   unsigned long long* bkey, uint32_t* btrail, hipStream_t s)
-{
-    const int nv = spec.nvcols;
-    if (nv <= 1) launch_seg<ITEMS, 1>(keys, kb, is_signed, spec, perm, n, head_off, ntiles, os, bstate, bkey, btrail, s);
-    else if (nv <= 2) launch_seg<ITEMS, 2>(keys, kb, is_signed, spec, perm, n, head_off, ntiles, os, bstate, bkey, btrail, s);
-    else if (nv <= 4) launch_seg<ITEMS, 4>(keys, kb, is_signed, spec, perm, n, head_off, ntiles, os, bstate, bkey, btrail, s);
-    else launch_seg<ITEMS, 8>(keys, kb, is_signed, spec, perm, n, head_off, ntiles, os, bstate, bkey, btrail, s);
+    const size_t lds = (size_t)spec.nops * (kSegB + 1) * 8;
+    auto go = [&](auto kern) {
+        if (enable_dynamic_lds((const void*)kern) < lds) raise(EG_ERR_UNSUPPORTED, "segmented reduction slots too large");
+        KernelTimer timer("segreduce", s);
+        hipLaunchKernelGGL(kern, dim3(ntiles), dim3(kSegB), lds, s, keys, kb, is_signed, spec, perm, n, head_off, ntiles,
+                           os, bstate, bkey, btrail);
+        EG_LAUNCH_CHECK();
+    };
+    with_vlayout(spec, true, [&](auto vl) {
+        using VL = decltype(vl);
+        if (kb == 4) go(seg_reduce_kernel<ITEMS, 4, VL>);
+        else go(seg_reduce_kernel<ITEMS, 8, VL>);
+    });
 }
 
 }  // namespace
@@ -253,10 +357,7 @@ Result reduce_sorted(const Plan& plan, const void* sorted_keys, const AggSpec& s
 {
     const int64_t n = plan.rows;
     const int kb = plan.key.bytes;
-    // tile size from the LDS the per-tile states need: (nops + 1) * T * 8 bytes
-    const size_t max_lds = (size_t)device_info().max_lds - 256;  // static LDS of the kernel
-    int items = 16;
-    while (items > 4 && (size_t)(spec.nops + 1) * kSegB * items * 8 > max_lds) items >>= 1;
+    constexpr int items = 16;
     const int T = kSegB * items;
     const int64_t ntiles64 = (n + T - 1) / T;
     if (ntiles64 >= ((int64_t)1 << 31)) raise(EG_ERR_UNSUPPORTED, "too many rows for the segmented reduction");
@@ -266,9 +367,8 @@ Result reduce_sorted(const Plan& plan, const void* sorted_keys, const AggSpec& s
     fill_zero(heads.get(), (size_t)(ntiles + 1) * 4, s);
     {
         KernelTimer timer("seg_heads", s);
-        if (items == 16) hipLaunchKernelGGL(seg_heads_kernel<16>, dim3(ntiles), dim3(kSegB), 0, s, sorted_keys, kb, n, heads.get<uint32_t>());
-        else if (items == 8) hipLaunchKernelGGL(seg_heads_kernel<8>, dim3(ntiles), dim3(kSegB), 0, s, sorted_keys, kb, n, heads.get<uint32_t>());
-        else hipLaunchKernelGGL(seg_heads_kernel<4>, dim3(ntiles), dim3(kSegB), 0, s, sorted_keys, kb, n, heads.get<uint32_t>());
+        hipLaunchKernelGGL(seg_heads_kernel<items>, dim3(ntiles), dim3(kSegB), 0, s, sorted_keys, kb, n,
+                           heads.get<uint32_t>());
         EG_LAUNCH_CHECK();
     }
     // head_off[t] = groups whose head lies before tile t; head_off[ntiles] = G
@@ -281,9 +381,8 @@ Result reduce_sorted(const Plan& plan, const void* sorted_keys, const AggSpec& s
     fill_zero(bstate.get(), (size_t)ntiles * spec.nops * 8 + 8, s);
     fill_zero(btrail.get(), (size_t)ntiles * 4, s);
     const int sg = plan.key.is_signed;
-    if (items == 16) launch_seg_nv<16>(sorted_keys, kb, sg, spec, perm, n, head_off.get<uint32_t>(), ntiles, os, bstate.get<unsigned long long>(), bkey.get<unsigned long long>(), btrail.get<uint32_t>(), s);
-    else if (items == 8) launch_seg_nv<8>(sorted_keys, kb, sg, spec, perm, n, head_off.get<uint32_t>(), ntiles, os, bstate.get<unsigned long long>(), bkey.get<unsigned long long>(), btrail.get<uint32_t>(), s);
-    else launch_seg_nv<4>(sorted_keys, kb, sg, spec, perm, n, head_off.get<uint32_t>(), ntiles, os, bstate.get<unsigned long long>(), bkey.get<unsigned long long>(), btrail.get<uint32_t>(), s);
+    launch_seg<items>(sorted_keys, kb, sg, spec, perm, n, head_off.get<uint32_t>(), ntiles, os,
+                      bstate.get<unsigned long long>(), bkey.get<unsigned long long>(), btrail.get<uint32_t>(), s);
     hipLaunchKernelGGL(seg_tail_kernel, dim3((ntiles + 255) / 256), dim3(256), 0, s, ntiles,
                        (const uint32_t*)head_off.get<uint32_t>(), (const unsigned long long*)bstate.get<unsigned long long>(),
                        (const unsigned long long*)bkey.get<unsigned long long>(), (const uint32_t*)btrail.get<uint32_t>(),

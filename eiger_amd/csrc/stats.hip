@@ -51,6 +51,36 @@ __global__ void hll_merge_kernel(const uint8_t* __restrict__ partial, int parts,
 
 }  // namespace
 
+namespace {
+double hll_estimate(const uint8_t* reg)
+{
+    double sum = 0.0;
+    int zeros = 0;
+    for (int i = 0; i < kM; ++i) {
+        sum += std::ldexp(1.0, -(int)reg[i]);
+        zeros += reg[i] == 0;
+    }
+    const double alpha = 0.7213 / (1.0 + 1.079 / kM);
+    double e = alpha * (double)kM * (double)kM / sum;
+    if (e <= 2.5 * kM && zeros > 0) e = (double)kM * std::log((double)kM / (double)zeros);
+    return e;
+}
+
+// Registers of one column into regs (kM bytes, device); asynchronous.
+void hll_registers(const void* data, int bytes, int64_t n, uint8_t* regs, DevBuf& partial, hipStream_t s)
+{
+    const DeviceInfo& di = device_info();
+    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>((n + 4095) / 4096, (int64_t)di.num_cus * 2));
+    partial = DevBuf((size_t)parts * kM, s);
+    KernelTimer timer("hll", s);
+    hipLaunchKernelGGL(hll_kernel, dim3(parts), dim3(256), 0, s, data, bytes, n, partial.get<uint8_t>());
+    EG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(hll_merge_kernel, dim3(kM / 256), dim3(256), 0, s, (const uint8_t*)partial.get<uint8_t>(), parts,
+                       regs);
+    EG_LAUNCH_CHECK();
+}
+}  // namespace
+
 int64_t ndv_approx(const void* data, int32_t type, int64_t n, hipStream_t s)
 {
     const int bytes = type_size(type);
@@ -70,22 +100,49 @@ int64_t ndv_approx(const void* data, int32_t type, int64_t n, hipStream_t s)
     std::vector<uint8_t> reg(kM);
     EG_HIP(hipMemcpyAsync(reg.data(), pp + (size_t)parts * kM, kM, hipMemcpyDeviceToHost, s));
     EG_HIP(hipStreamSynchronize(s));
-    double sum = 0.0;
-    int zeros = 0;
-    for (int i = 0; i < kM; ++i) {
-        sum += std::ldexp(1.0, -(int)reg[i]);
-        zeros += reg[i] == 0;
-    }
-    const double alpha = 0.7213 / (1.0 + 1.079 / kM);
-    double e = alpha * (double)kM * (double)kM / sum;
-    if (e <= 2.5 * kM && zeros > 0) e = (double)kM * std::log((double)kM / (double)zeros);
-    int64_t r = (int64_t)std::llround(e);
+    int64_t r = (int64_t)std::llround(hll_estimate(reg.data()));
     return std::max<int64_t>(1, std::min<int64_t>(r, n));
 }
 
 }  // namespace eg
 
 extern "C" {
+
+// The rule for many key columns with one synchronisation: every column's
+// HyperLogLog runs asynchronously, the registers come back in one copy.
+int eg_auto_select_algo_batch(const eg_column_view* keys, int64_t num, int32_t* algos, int64_t* ndv, void* stream)
+{
+    return eg::guard([&] {
+        if (num < 0 || (num > 0 && (!keys || !algos))) eg::raise(EG_ERR_INVALID, "bad arguments");
+        if (num == 0) return;
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        eg::DevBuf regs((size_t)num * eg::kM, s);
+        std::vector<eg::DevBuf> partials((size_t)num);
+        for (int64_t i = 0; i < num; ++i) {
+            const eg_column_view& c = keys[i];
+            const int bytes = eg::type_size(c.type);
+            if (!bytes || c.type == EG_STRING) eg::raise(EG_ERR_UNSUPPORTED, "unsupported key type");
+            if (c.size > 0 && !c.data) eg::raise(EG_ERR_INVALID, "null column data");
+            uint8_t* r = regs.get<uint8_t>() + (size_t)i * eg::kM;
+            if (c.size <= 0) {
+                eg::fill_zero(r, eg::kM, s);
+                continue;
+            }
+            eg::hll_registers((const char*)c.data + c.offset * bytes, bytes, c.size, r, partials[(size_t)i], s);
+        }
+        std::vector<uint8_t> h((size_t)num * eg::kM);
+        EG_HIP(hipMemcpyAsync(h.data(), regs.get(), h.size(), hipMemcpyDeviceToHost, s));
+        EG_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < num; ++i) {
+            const int64_t n = keys[i].size;
+            const int64_t est = n <= 0 ? 0
+                : std::max<int64_t>(1, std::min<int64_t>((int64_t)std::llround(eg::hll_estimate(h.data() + (size_t)i * eg::kM)), n));
+            if (ndv) ndv[i] = est;
+            const int64_t thr = eg::type_size(keys[i].type) == 4 ? ((int64_t)1 << 14) : ((int64_t)1 << 18);
+            algos[i] = est <= thr ? EG_ALGO_HASH : EG_ALGO_PARTITION;
+        }
+    });
+}
 
 int eg_compute_ndv_approx(eg_column_view col, void* stream, int64_t* out_ndv)
 {

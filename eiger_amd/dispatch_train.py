@@ -132,32 +132,68 @@ def soft_targets(times: np.ndarray, tau: float) -> np.ndarray:
     return (w / w.sum(axis=1, keepdims=True)).astype(np.float32)
 
 
-def evaluate(params: np.ndarray, tr: dict, infer_ms: float) -> dict:
+def choice_ms(params: np.ndarray, tr: dict) -> float:
+    y = G.mlp_forward(params, tr["features"])
+    pick = np.argmax(y[:, :3], axis=1)
+    return float(tr["times"][np.arange(len(pick)), pick].sum())
+
+
+def evaluate(params: np.ndarray, tr: dict, mlp_overhead_ms: float, rule_overhead_ms: float) -> dict:
+    """Totals of the strategies each dispatcher picks plus its own cost per
+    query (mlp/rule_overhead_ms: per-query dispatch cost)."""
     x, t = tr["features"], tr["times"]
     n = len(x)
     y = G.mlp_forward(params, x)
     pick = np.argmax(y[:, :3], axis=1)
     best = t.min(axis=1)
-    mlp_total = float(t[np.arange(n), pick].sum() + tr["feat_ms"].sum())
-    rule_total = float(t[np.arange(n), tr["rule"]].sum() + tr["rule_ms"].sum())
-    out = {
+    mlp_choice = float(t[np.arange(n), pick].sum())
+    rule_choice = float(t[np.arange(n), tr["rule"]].sum())
+    mlp_total = mlp_choice + mlp_overhead_ms * n
+    rule_total = rule_choice + rule_overhead_ms * n
+    return {
         "queries": n,
         "best_ms": float(best.sum()),
         "mlp_ms": mlp_total,
         "rule_ms": rule_total,
-        "mlp_choice_only_ms": float(t[np.arange(n), pick].sum()),
-        "rule_choice_only_ms": float(t[np.arange(n), tr["rule"]].sum()),
-        "mlp_overhead_ms": float(tr["feat_ms"].sum()),
-        "rule_overhead_ms": float(tr["rule_ms"].sum()),
+        "mlp_choice_only_ms": mlp_choice,
+        "rule_choice_only_ms": rule_choice,
+        "mlp_overhead_ms_per_query": mlp_overhead_ms,
+        "rule_overhead_ms_per_query": rule_overhead_ms,
         "fixed_ms": {s: float(t[:, j].sum()) for j, s in enumerate(STRATS)},
         "mlp_over_best": mlp_total / float(best.sum()),
+        "mlp_choice_over_best": mlp_choice / float(best.sum()),
         "rule_over_mlp": rule_total / mlp_total,
+        "rule_choice_over_mlp_choice": rule_choice / mlp_choice,
         "mlp_picks_best": float((pick == t.argmin(axis=1)).mean()),
         "rule_picks_best": float((tr["rule"] == t.argmin(axis=1)).mean()),
         "mlp_within_10pct": float((t[np.arange(n), pick] <= 1.1 * best).mean()),
         "card_log2_mae": float(np.abs(y[:, 3] * 32 - np.log2(np.maximum(tr["groups"], 1))).mean()),
     }
-    return out
+
+
+def batched_overheads(specs, seed0: int, wl: Workload, chunk: int = 512):
+    """Per-query dispatch cost when a batch of queries is dispatched at once
+    (the batched workload): the MLP's one feature + forward launch for the
+    chunk against the rule's per-column HyperLogLog with one synchronisation.
+    Returns (mlp_ms_per_query, rule_ms_per_query)."""
+    mlp_ms = rule_ms = 0.0
+    done = 0
+    for c0 in range(0, len(specs), chunk):
+        part = specs[c0:c0 + chunk]
+        qs = [wl.query(sp, seed0 + c0 + i) for i, sp in enumerate(part)]
+        keys = [q[0] for q in qs]
+        nops = [q[3] for q in qs]
+        vb = [q[4] for q in qs]
+        G.mlp_select_batch(keys[:4], None, nops[:4], vb[:4])  # warm
+        G.auto_select_groupby_algo_batch(keys[:4])
+        _, a = timed(G.mlp_select_batch, keys, None, nops, vb)
+        _, b = timed(G.auto_select_groupby_algo_batch, keys)
+        mlp_ms += a
+        rule_ms += b
+        done += len(part)
+        for k in keys:
+            k.free()
+    return mlp_ms / done, rule_ms / done
 
 
 def main():
@@ -165,7 +201,6 @@ def main():
     p.add_argument("--train", type=int, default=50000)
     p.add_argument("--heldout", type=int, default=10000)
     p.add_argument("--epochs", type=int, default=80)
-    p.add_argument("--tau", type=float, default=0.05)
     p.add_argument("--out", default="dispatch_out")
     args = p.parse_args()
     os.makedirs(args.out, exist_ok=True)
@@ -185,23 +220,42 @@ def main():
     log("tracing held-out queries")
     ho = trace(specs_ho, 2_000_000, wl)
 
-    target = soft_targets(tr["times"], args.tau)
     card = (np.log2(np.maximum(tr["groups"], 1)) / 32).astype(np.float32)
-    P0 = G.mlp_init(0)
-    a = time.perf_counter()
+    # model selection on a validation split of the training trace: the soft
+    # target temperature and the epoch count that pick the fastest strategies
+    nfit = int(0.8 * len(card))
+    fit = {k: v[:nfit] for k, v in tr.items()}
+    val = {k: v[nfit:] for k, v in tr.items()}
+    batch = max(64, min(1024, (nfit // 64) * 64))
+    grid = []
+    for tau in (0.01, 0.03, 0.1):
+        for epochs in (args.epochs, 3 * args.epochs):
+            p_, _ = G.mlp_train(G.mlp_init(0), fit["features"], soft_targets(fit["times"], tau), card[:nfit],
+                                epochs=epochs, batch=batch, lr=2e-3, lam=4.0, weight_decay=1e-5, seed=3)
+            grid.append((choice_ms(p_, val), tau, epochs))
+            log(f"tau={tau} epochs={epochs}: validation choice time {grid[-1][0]:.1f} ms")
+    _, tau, epochs = min(grid)
+    target = soft_targets(tr["times"], tau)
     batch = max(64, min(1024, (len(target) // 64) * 64))
-    params, losses = G.mlp_train(P0, tr["features"], target, card, epochs=args.epochs, batch=batch, lr=2e-3,
+    a = time.perf_counter()
+    params, losses = G.mlp_train(G.mlp_init(0), tr["features"], target, card, epochs=epochs, batch=batch, lr=2e-3,
                                  lam=4.0, weight_decay=1e-5, seed=3)
     train_s = time.perf_counter() - a
-    log(f"trained {args.epochs} epochs in {train_s:.1f}s: loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    log(f"trained tau={tau} {epochs} epochs in {train_s:.1f}s: loss {losses[0]:.4f} -> {losses[-1]:.4f}")
 
+    G.mlp_load(params)
+    bm, br = batched_overheads(specs_ho[:2048], 2_000_000, wl)
     G.mlp_load(None)
-    infer_ms = float(np.median(ho["feat_ms"]))  # features + inference per query (select path)
-
-    ev = {"train": evaluate(params, tr, infer_ms), "heldout": evaluate(params, ho, infer_ms),
-          "epochs": args.epochs, "tau": args.tau, "train_seconds": train_s,
-          "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]), "infer_ms": infer_ms}
-    log(json.dumps(ev["heldout"], indent=1))
+    sm = float(np.median(ho["feat_ms"]))  # single-query select path
+    sr = float(np.median(ho["rule_ms"]))  # single-query rule (HyperLogLog + sync)
+    ev = {"heldout_batched": evaluate(params, ho, bm, br),
+          "heldout_single_query": evaluate(params, ho, sm, sr),
+          "train_batched": evaluate(params, tr, bm, br),
+          "selection": [{"validation_choice_ms": g[0], "tau": g[1], "epochs": g[2]} for g in grid],
+          "epochs": epochs, "tau": tau, "train_seconds": train_s, "train_queries": len(card),
+          "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])}
+    log(json.dumps(ev["heldout_batched"], indent=1))
+    log(json.dumps(ev["heldout_single_query"], indent=1))
     params.astype("<f4").tofile(os.path.join(args.out, "dispatch_mlp.f32"))
     with open(os.path.join(args.out, "dispatch_eval.json"), "w") as f:
         json.dump(ev, f, indent=1)

@@ -14,6 +14,7 @@ no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 from enum import IntEnum
 from typing import Iterable, List, Optional, Sequence
@@ -682,3 +683,54 @@ def mlp_select(key: column_view, group_bound=0, nops=2, value_bytes=8, stream=No
     _check(lib().eg_mlp_select(key._to_c(), int(group_bound), int(nops), int(value_bytes), _stream(stream),
                                C.byref(a), C.byref(card)))
     return groupby_algo(a.value), card.value
+
+
+def _mlp_queries(keys, group_bounds, nops, value_bytes):
+    n = len(keys)
+    arr = (_lib_mod.MlpQuery * max(n, 1))()
+    for i, k in enumerate(keys):
+        arr[i].keys = (k.data + k.offset * size_of(k.type)) if k.data else None
+        arr[i].rows = k.size
+        arr[i].group_bound = int(group_bounds[i]) if group_bounds is not None else 0
+        arr[i].key_type = k.type
+        arr[i].nops = int(nops[i]) if hasattr(nops, "__len__") else int(nops)
+        arr[i].value_bytes = int(value_bytes[i]) if hasattr(value_bytes, "__len__") else int(value_bytes)
+    return arr
+
+
+def mlp_select_batch(keys: Sequence[column_view], group_bounds=None, nops=2, value_bytes=8, stream=None):
+    """The loaded model's strategies and group-count estimates for many keys
+    (one feature launch, one forward launch, one synchronisation)."""
+    n = len(keys)
+    algos = np.zeros(max(n, 1), np.int32)
+    card = np.zeros(max(n, 1), np.int64)
+    _check(lib().eg_mlp_select_batch(_mlp_queries(keys, group_bounds, nops, value_bytes), n,
+                                     algos.ctypes.data_as(C.c_void_p), card.ctypes.data_as(C.c_void_p),
+                                     _stream(stream)))
+    return algos[:n], card[:n]
+
+
+def auto_select_groupby_algo_batch(keys: Sequence[column_view], stream=None):
+    """The reference rule for many keys with one synchronisation."""
+    n = len(keys)
+    algos = np.zeros(max(n, 1), np.int32)
+    ndv = np.zeros(max(n, 1), np.int64)
+    _check(lib().eg_auto_select_algo_batch(_views(keys), n, algos.ctypes.data_as(C.c_void_p),
+                                           ndv.ctypes.data_as(C.c_void_p), _stream(stream)))
+    return algos[:n], ndv[:n]
+
+
+DEFAULT_DISPATCH_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "dispatch_mlp.f32")
+
+
+def load_dispatch_model(path: Optional[str] = None) -> bool:
+    """Loads the trained dispatch model (default: the one shipped with the
+    package) so that groupby() with AUTO uses it; False if the file is absent."""
+    path = path or DEFAULT_DISPATCH_MODEL
+    if not os.path.exists(path):
+        return False
+    p = np.fromfile(path, dtype="<f4")
+    if p.size != MLP_PARAMS:
+        raise GamaxError(EG_ERR_INVALID, f"{path}: expected {MLP_PARAMS} float32 parameters, found {p.size}")
+    mlp_load(p)
+    return True

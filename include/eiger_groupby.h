@@ -166,11 +166,15 @@ int eg_table_release_column(eg_table t, int64_t i, eg_column_view* out);
 void eg_table_free(eg_table t);
 
 /* ---- device memory (gamax::MemoryManager, memory/mem_manager.hpp:12-158) -- */
-enum { EG_MM_ASYNC = 0, EG_MM_POOL = 1 };
+enum { EG_MM_ASYNC = 0, EG_MM_POOL = 1, EG_MM_CACHE = 2 };
 int eg_malloc(void** ptr, int64_t bytes, int32_t clear, void* stream);
 int eg_free(void* ptr);
 int eg_memory_usage(int64_t* current_bytes, int64_t* peak_bytes);
-/* EG_MM_ASYNC: stream-ordered allocations (the reference's CudaMM);
+/* EG_MM_CACHE (default): blocks from hipMalloc in size classes (four per
+ * power of two), kept after eg_free and reused by later allocations on the
+ * same stream (stream order makes the reuse safe); cached blocks are returned
+ * to HIP only when an allocation would otherwise fail.
+ * EG_MM_ASYNC: stream-ordered hipMallocAsync (the reference's CudaMM);
  * EG_MM_POOL: one slab of pool_bytes with a first-fit free list, 256-B
  * aligned (the reference's UserSpaceMM).  Only while nothing is allocated. */
 int eg_set_memory_mode(int32_t mode, int64_t pool_bytes);
@@ -230,6 +234,11 @@ int eg_compute_ndv_approx(eg_column_view col, void* stream, int64_t* out_ndv);
  * hint->groupby_algo in place when it is EG_ALGO_AUTO. */
 int eg_auto_select_algo(const eg_column_view* keys, int64_t num_keys,
                         eg_groupby_hint* hint, void* stream);
+/* The same rule for `num` independent single-column keys with one
+ * synchronisation (no reference counterpart: batched callers); algos[i] gets
+ * EG_ALGO_HASH or EG_ALGO_PARTITION, ndv[i] (optional) the estimate. */
+int eg_auto_select_algo_batch(const eg_column_view* keys, int64_t num, int32_t* algos, int64_t* ndv,
+                              void* stream);
 
 /* ---- batched group-by (no reference counterpart) -----------------------------
  * Many independent single-key group-bys executed by one persistent launch.
@@ -336,6 +345,10 @@ int eg_mlp_loaded(void);
  * estimate for one key column. */
 int eg_mlp_select(eg_column_view key, int64_t group_bound, int32_t nops, int32_t value_bytes, void* stream,
                   int32_t* algo, int64_t* card_est);
+/* The loaded model's choice for `num` queries: one feature launch, one
+ * forward launch, one synchronisation. */
+int eg_mlp_select_batch(const eg_mlp_query* queries, int64_t num, int32_t* algos, int64_t* card_est,
+                        void* stream);
 
 /* ---- kernel timing (measurement support; no reference counterpart) -------- */
 int eg_kernel_timing_enable(int32_t on);

@@ -101,3 +101,29 @@ def test_dispatch_choice_never_changes_results():
         t.free()
     G.mlp_load(None)
     assert not G.mlp_loaded()
+
+
+def test_batched_selection_equals_single_query_selection():
+    rng = np.random.default_rng(21)
+    keys = []
+    for i, (rows, d) in enumerate([(1000, 10), (50_000, 40_000), (300_000, 1 << 20), (70_000, 3), (0, 1)]):
+        k = rng.integers(0, d, rows).astype(np.int32 if i % 2 else np.int64)
+        keys.append(G.make_column_from_host(k))
+    ra, rn = G.auto_select_groupby_algo_batch(keys)
+    for k, a, nd in zip(keys, ra, rn):
+        if k.size == 0:
+            assert nd == 0 and a == G.groupby_algo.HASH
+            continue
+        h = G.groupby_hint()
+        G.auto_select_groupby_algo([k], h)
+        assert h.groupby_algo == a and nd == G.compute_ndv_approx(k)
+    P = G.mlp_init(4)
+    G.mlp_load(P)
+    try:
+        live = [k for k in keys if k.size > 0]
+        ma, mc = G.mlp_select_batch(live, None, 2, 8)
+        for k, a, c in zip(live, ma, mc):
+            a1, c1 = G.mlp_select(k, 0, 2, 8)
+            assert a1 == a and c1 == c
+    finally:
+        G.mlp_load(None)
