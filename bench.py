@@ -39,6 +39,9 @@ STRATS = {"auto": "groupby", "hash": "hash_groupby", "sort": "sort_groupby", "so
 # the key and both values once (2 x 16 B).
 KERNELS = ("hash_agg", "hash_compact", "hll", "key_range", "radix_hist", "radix_scan", "radix_scatter", "seg_heads",
            "segreduce", "partition_bounds", "partition_agg", "partition_compact", "fill_zero", "mlp_select")
+# timer name -> kernel function name (rocprofv3 reports the function)
+TIMER_KERNEL = {"partition_agg": "part_agg_kernel", "segreduce": "seg_reduce_kernel",
+                "partition_bounds": "part_bounds_kernel", "partition_compact": "part_compact_kernel"}
 KERNEL_BYTES_PER_ROW = {"hash_agg": 16, "partition_agg": 16, "segreduce": 16, "radix_scatter": 32,
                         "radix_hist": 4, "seg_heads": 4, "key_range": 4, "hll": 4, "partition_bounds": 4}
 
@@ -56,6 +59,7 @@ def parse():
     p.add_argument("--strategy", default="auto", choices=list(STRATS))
     p.add_argument("--explore", action="store_true")
     p.add_argument("--rule-only", action="store_true", help="auto dispatch without the learned model")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched uneven-query workload")
     p.add_argument("--cases", default="all", help="'all' or comma list like 16:uniform,1024:zipf1.0")
     p.add_argument("--cpu-sample-rows", type=int, default=1 << 22)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -204,13 +208,28 @@ def main():
         avg_s = ms / cnt * 1e-3
         per_launch = KERNEL_BYTES_PER_ROW.get(dom, 16) * n  # every launch covers the n rows of one case
         achieved = per_launch / avg_s / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+        if os.path.exists(tf):
+            # measured HBM bytes per launch of this kernel on this workload
+            # (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by the copy kernel)
+            for kname, rec in json.load(open(tf)).get("kernels", {}).items():
+                if kname.startswith(TIMER_KERNEL.get(dom, dom + "_kernel")):
+                    traffic = int(rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"])
+                    break
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9,
-                    "unit": "GB/s", "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": traffic,
                     "launches": cnt, "avg_ms": round(ms / cnt, 4), "bytes_per_row": KERNEL_BYTES_PER_ROW.get(dom, 16),
                     "bytes_per_launch": per_launch, "copy_probe_gbs": round(copy_gbs, 1)}
         log(f"dominant kernel {dom}: {cnt} launches, avg {ms / cnt:.3f} ms, {achieved:.0f} GB/s algorithmic")
         for k, (c, m) in kernels.items():
             log(f"  kernel {k:16s} launches={c:4d} total={m:9.3f} ms")
+
+    batched = None
+    if not args.no_batch:
+        batched = batched_workload(G)
+        log(f"batched: {batched['grows_per_s']} Grows/s, {batched['ms']} ms for {batched['queries']} queries "
+            f"({batched['speedup_vs_sequential']}x the sequential calls)")
 
     # ---- CPU baseline (oracle, OpenMP) on a bounded sample -------------------------
     cpu = None
@@ -232,9 +251,79 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "dispatch": dispatch,
+        "batched": batched,
         "sweep": sweep,
     }
     print(json.dumps(out), flush=True)
+
+
+def batched_workload(G, nq=4096, seed=42):
+    """Batched uneven queries: nq independent group-bys (rows log-uniform in
+    [1e3, 4e6], distinct count log-uniform in [1, rows/2], int32 keys uniform
+    or Zipf s ~ U[0.5, 1.5], int64 values; SUM + COUNT) in one persistent
+    launch replayed from a hipGraph, tuned by eg_batch_autotune.  Compared
+    with the same queries run one by one through the top-level groupby."""
+    import math
+
+    import numpy as np
+
+    A = G.aggregation_id
+    T = G.type_id
+    rng = np.random.default_rng(seed)
+    specs = []
+    for q in range(nq):
+        rows = int(math.exp(rng.uniform(math.log(1e3), math.log(4e6))))
+        d = max(1, int(math.exp(rng.uniform(0.0, math.log(max(rows / 2, 1.0))))))
+        s = 0.0 if rng.random() < 0.5 else float(rng.uniform(0.5, 1.5))
+        specs.append((rows, d, s))
+    total = sum(r for r, _, _ in specs)
+    t0 = time.time()
+    vals = G.gen_values(total, G.val_dist.UNIFORM_I64, T.INT64, seed=seed, stream_id=2)
+    keys, outs, queries = [], [], []
+    off = 0
+    for q, (rows, d, s) in enumerate(specs):
+        k = G.gen_keys_uniform(rows, d, seed=seed + q, t=T.INT32) if s == 0.0 else \
+            G.gen_keys_zipf(rows, d, s, seed=seed + q, t=T.INT32)
+        keys.append(k)
+        cap = min(rows, d)
+        ok, o1, o2 = G.make_column(T.INT32, cap), G.make_column(T.INT64, cap), G.make_column(T.INT64, cap)
+        outs += [ok, o1, o2]
+        queries.append(G.batch_query(k, G.column_view(vals.data + off * 8, rows, T.INT64), d, ok, [o1, o2]))
+        off += rows
+    log(f"batched workload: {nq} queries, {total} rows generated in {time.time() - t0:.1f}s")
+    reqs = [(0, A.SUM), (-1, A.COUNT_ALL)]
+    cands = [G.batch_tuning(tile_rows_log2=t, lds_slots_log2=c, lds_max_bound_log2=b, blocks_per_cu=w)
+             for t in (15, 17) for c in (10, 11) for b in (10, 12) for w in (1, 2)]
+    best, times = G.batch_autotune(queries, T.INT32, T.INT64, reqs, cands, reps=1)
+    plan = G.batch_plan(queries, T.INT32, T.INT64, reqs, tuning=best)
+    plan.graph_launch()
+    ng, st = plan.results()
+    reps = 5
+    a = time.perf_counter()
+    for _ in range(reps):
+        plan.graph_launch()
+    plan.results()
+    batch_s = (time.perf_counter() - a) / reps
+    # the same queries one by one (top-level groupby, rule dispatch, bound hint)
+    G.mlp_load(None)
+    a = time.perf_counter()
+    for q, (rows, d, s) in enumerate(specs):
+        t = G.groupby([keys[q]], [queries[q].vals], reqs, G.groupby_hint(groups_upper_bound=d))
+        t.free()
+    seq_s = time.perf_counter() - a
+    bad = int(np.count_nonzero(st))
+    for c in keys + outs:
+        c.free()
+    vals.free()
+    plan.free()
+    bt = best
+    return {"queries": nq, "rows": total, "grows_per_s": round(total / batch_s / 1e9, 3),
+            "ms": round(batch_s * 1e3, 3), "sequential_ms": round(seq_s * 1e3, 1),
+            "speedup_vs_sequential": round(seq_s / batch_s, 2), "queries_with_status": bad,
+            "algorithmic_hbm_frac": round((12 * total + 20 * int(ng.sum())) / batch_s / PEAK_HBM, 4),
+            "tuning": {"tile_rows_log2": bt.tile_rows_log2, "lds_slots_log2": bt.lds_slots_log2,
+                       "lds_max_bound_log2": bt.lds_max_bound_log2, "blocks_per_cu": bt.blocks_per_cu},
+            "autotune_candidates": len(cands), "autotune_ms": [round(float(x), 3) for x in times]}
 
 
 def cpu_baseline(cases, n, sample):

@@ -114,6 +114,9 @@ private:
 
 // Pinned host scratch for small device->host reads (one per thread).
 int64_t* pinned_i64(int count);
+// Thread-local pinned staging buffer of at least `bytes` (reused; contents
+// valid until the next call on this thread with slot `slot`).
+void* pinned_staging(size_t bytes, int slot);
 
 // ---- device facts ------------------------------------------------------------
 struct DeviceInfo {

@@ -83,16 +83,18 @@ __device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* red)
     return t;
 }
 
+constexpr int kFB = 1024;  // threads of the feature workgroup
+
 struct FeatLds {
     unsigned long long skey[kFSlots];
     uint32_t scnt[kFSlots];
-    float redf[4];
-    uint32_t redu[4];
+    float redf[kFB / 64];
+    uint32_t redu[kFB / 64];
     uint32_t s_asc;
     unsigned long long s_min, s_max;
 };
 
-// The 16 features of query q, written by thread 0 to o (256 threads).
+// The 16 features of query q, written by thread 0 to o (kFB threads).
 __device__ void compute_features(const FQuery& q, FeatLds& L, float* o)
 {
     unsigned long long* skey = L.skey;
@@ -119,15 +121,31 @@ __device__ void compute_features(const FQuery& q, FeatLds& L, float* o)
     // a key k is stored as k + 1 so that 0 marks an empty slot (the one key
     // with k + 1 == 0 takes a fixed stand-in tag)
     uint32_t asc = 0;
-    for (int i = t; i < ns; i += blockDim.x) {
-        const int64_t r = (int64_t)i * stride;
-        const uint64_t raw = dev::load_raw(q.keys, q.key_bytes, r);
-        const uint64_t k = q.key_bytes == 4 ? (uint64_t)(int64_t)(int32_t)(uint32_t)raw : raw;
-        if (i + 1 < ns) {
-            const uint64_t raw2 = dev::load_raw(q.keys, q.key_bytes, r + stride);
-            const uint64_t k2 = q.key_bytes == 4 ? (uint64_t)(int64_t)(int32_t)(uint32_t)raw2 : raw2;
-            asc += ((int64_t)k2 >= (int64_t)k) ? 1u : 0u;
+    // the sample's keys (and each one's successor) are loaded before any is
+    // inserted, so the loads overlap
+    constexpr int kPer = kSample / kFB;
+    uint64_t ks[kPer], kn[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int i = t + j * kFB;
+        ks[j] = kn[j] = 0ull;
+        if (i < ns) {
+            const int64_t r = (int64_t)i * stride;
+            const uint64_t raw = q.key_bytes == 4 ? (uint64_t)((const uint32_t*)q.keys)[r] : ((const uint64_t*)q.keys)[r];
+            ks[j] = q.key_bytes == 4 ? (uint64_t)(int64_t)(int32_t)(uint32_t)raw : raw;
+            if (i + 1 < ns) {
+                const uint64_t raw2 = q.key_bytes == 4 ? (uint64_t)((const uint32_t*)q.keys)[r + stride]
+                                                       : ((const uint64_t*)q.keys)[r + stride];
+                kn[j] = q.key_bytes == 4 ? (uint64_t)(int64_t)(int32_t)(uint32_t)raw2 : raw2;
+            }
         }
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int i = t + j * kFB;
+        if (i >= ns) break;
+        const uint64_t k = ks[j];
+        if (i + 1 < ns) asc += ((int64_t)kn[j] >= (int64_t)k) ? 1u : 0u;
         const uint64_t ek = k ^ 0x8000000000000000ull;  // order-preserving for the span
         atomicMin(&s_min, (unsigned long long)ek);
         atomicMax(&s_max, (unsigned long long)ek);
@@ -208,8 +226,8 @@ __device__ void compute_features(const FQuery& q, FeatLds& L, float* o)
     o[15] = nsf / (float)kSample;
 }
 
-// 256 threads per query.
-__global__ __launch_bounds__(256) void features_kernel(const FQuery* __restrict__ qs, float* __restrict__ out)
+// kFB threads per query.
+__global__ __launch_bounds__(kFB) void features_kernel(const FQuery* __restrict__ qs, float* __restrict__ out)
 {
     __shared__ FeatLds L;
     const FQuery q = qs[blockIdx.x];
@@ -218,7 +236,7 @@ __global__ __launch_bounds__(256) void features_kernel(const FQuery* __restrict_
 
 // Features of one query and the MLP's four outputs in one launch (the
 // dispatcher's path: one kernel, one 16-byte copy back).
-__global__ __launch_bounds__(256) void select_kernel(FQuery q, const float* __restrict__ P, float* __restrict__ y)
+__global__ __launch_bounds__(kFB) void select_kernel(FQuery q, const float* __restrict__ P, float* __restrict__ y)
 {
     __shared__ FeatLds L;
     __shared__ float x[kIn], h1[kH], h2[kH];
@@ -460,7 +478,7 @@ void mlp_features(const FQuery* dq, int64_t nq, float* dout, hipStream_t s)
 {
     if (nq <= 0) return;
     KernelTimer timer("mlp_features", s);
-    hipLaunchKernelGGL(features_kernel, dim3((unsigned)nq), dim3(256), 0, s, dq, dout);
+    hipLaunchKernelGGL(features_kernel, dim3((unsigned)nq), dim3(kFB), 0, s, dq, dout);
     EG_LAUNCH_CHECK();
 }
 
@@ -489,7 +507,7 @@ bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_
     FQuery q{keys, rows, group_bound, type_size(key_type), nops, value_bytes, 0};
     {
         KernelTimer timer("mlp_select", s);
-        hipLaunchKernelGGL(select_kernel, dim3(1), dim3(256), 0, s, q, (const float*)m.params.get<float>(),
+        hipLaunchKernelGGL(select_kernel, dim3(1), dim3(kFB), 0, s, q, (const float*)m.params.get<float>(),
                            m.out.get<float>());
         EG_LAUNCH_CHECK();
     }

@@ -328,8 +328,8 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
         else hipLaunchKernelGGL(part_bounds_kernel<8>, dim3((unsigned)grid), dim3(256), 0, s, pkeys, n, pmask, offs.get<int64_t>());
         EG_LAUNCH_CHECK();
     }
-    std::vector<int64_t> ho((size_t)(P + 1));
-    EG_HIP(hipMemcpyAsync(ho.data(), offs.get(), (size_t)(P + 1) * 8, hipMemcpyDeviceToHost, s));
+    int64_t* ho = (int64_t*)pinned_staging((size_t)(P + 1) * 8, 0);
+    EG_HIP(hipMemcpyAsync(ho, offs.get(), (size_t)(P + 1) * 8, hipMemcpyDeviceToHost, s));
     EG_HIP(hipStreamSynchronize(s));
     // Rows per unit: a partition is split when it is much larger than the
     // average (skew), or when there are too few partitions to occupy every CU,
@@ -340,7 +340,7 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     const int64_t slice = std::max<int64_t>({(int64_t)tune.block * 16, 8 * per_part, std::min(4 * avg_rows, balance)});
     std::vector<Unit> units;
     for (int64_t p = 0; p < P; ++p) {
-        const int64_t a = ho[(size_t)p], e = ho[(size_t)p + 1];
+        const int64_t a = ho[p], e = ho[p + 1];
         if (e <= a) continue;
         if (e - a <= slice) {
             units.push_back(Unit{a, e, 0, 0});
@@ -350,14 +350,19 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     }
     if (units.size() >= ((size_t)1 << 31)) raise(EG_ERR_UNSUPPORTED, "too many partition units");
     DevBuf dunits(units.size() * sizeof(Unit), s);
-    EG_HIP(hipMemcpyAsync(dunits.get(), units.data(), units.size() * sizeof(Unit), hipMemcpyHostToDevice, s));
+    {
+        // staged through pinned memory: the copy is a DMA, not a pageable bounce
+        void* hu = pinned_staging(units.size() * sizeof(Unit), 1);
+        std::memcpy(hu, units.data(), units.size() * sizeof(Unit));
+        EG_HIP(hipMemcpyAsync(dunits.get(), hu, units.size() * sizeof(Unit), hipMemcpyHostToDevice, s));
+    }
 
     phase_units.reset();
     HostPhase phase_agg("host_partition_aggregate", s);
     // ---- aggregation ------------------------------------------------------------------
     // HBM table: groups of sliced partitions plus LDS spills
     int64_t sliced_parts = 0;
-    for (int64_t p = 0; p < P; ++p) sliced_parts += (ho[(size_t)p + 1] - ho[(size_t)p]) > slice ? 1 : 0;
+    for (int64_t p = 0; p < P; ++p) sliced_parts += (ho[p + 1] - ho[p]) > slice ? 1 : 0;
     int64_t capacity = bound;
     int64_t slots = 1 << 14;
     // sliced partitions' groups, plus keys the LDS tables may not hold (a

@@ -309,6 +309,26 @@ int64_t* pinned_i64(int count)
     return buf;
 }
 
+void* pinned_staging(size_t bytes, int slot)
+{
+    struct Buf {
+        void* p = nullptr;
+        size_t cap = 0;
+    };
+    static thread_local Buf bufs[4];
+    if (slot < 0 || slot >= 4) raise(EG_ERR_INVALID, "pinned staging slot");
+    Buf& b = bufs[slot];
+    if (bytes > b.cap) {
+        if (b.p) (void)hipHostFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        const size_t c = std::max<size_t>(bytes + bytes / 2, 1 << 16);
+        EG_HIP(hipHostMalloc(&b.p, c, hipHostMallocDefault));
+        b.cap = c;
+    }
+    return b.p;
+}
+
 // ---- kernel timing -----------------------------------------------------------------
 namespace {
 struct TimingRec {
