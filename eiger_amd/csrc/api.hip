@@ -68,7 +68,6 @@ struct PlanBuilder {
 // expect; output columns are re-pointed at the moved ops.
 void order_ops_by_class(Plan& p)
 {
-    const uint64_t S = 0x8000000000000000ull;
     AggSpec& sp = p.spec;
     std::vector<int> order;
     for (int cls = 0; cls < 4; ++cls) {
@@ -84,16 +83,8 @@ void order_ops_by_class(Plan& p)
     for (int a = 0; a < sp.nops; ++a) old[a] = sp.ops[a];
     for (int i = 0; i < sp.nops; ++i) {
         OpDesc od = old[order[i]];
-        uint32_t xf = XF_NONE;
-        switch (od.code) {
-        case OPC_MIN_U: od.xmask = ~0ull; od.fmask = 0; xf = XF_MINU; break;
-        case OPC_MAX_I: od.xmask = S; od.fmask = 0; xf = XF_MAXI; break;
-        case OPC_MIN_I: od.xmask = ~S; od.fmask = 0; xf = XF_MINI; break;
-        case OPC_MAX_F: od.xmask = S; od.fmask = ~S; xf = XF_MAXF; break;
-        case OPC_MIN_F: od.xmask = ~S; od.fmask = ~S; xf = XF_MINF; break;
-        default: od.xmask = 0; od.fmask = 0; break;
-        }
-        od.packed = xf | (uint32_t)(od.col + 1) << 4 | (uint32_t)op_class(od.code) << 8;
+        od.xmask = op_xmask(od.code);
+        od.fmask = op_fmask(od.code);
         sp.ops[i] = od;
         where[order[i]] = i;
     }

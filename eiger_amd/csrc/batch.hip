@@ -153,8 +153,8 @@ __global__ __launch_bounds__(kBatchBlock) void batch_agg_kernel(const BQuery* __
                 cv[c] = (valid && spec.nvcols > 0) ? dev::canon(dev::load_raw(q.vals, vbytes, r), spec.cols[0].conv)
                                                    : 0ull;
             bool sp = valid;
-            if (use_lds) sp = lds_row<KB, NV>(tab, true, 0, lane, key, spec, kraw, cv, valid, probe_limit, peel_min);
-            if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kraw, cv, peel_min, ins);
+            if (use_lds) sp = lds_row<dev::RtOps, KB, NV>(tab, true, 0, lane, key, spec, kraw, cv, valid, probe_limit, peel_min);
+            if (__ballot(sp) != 0ull) hbm_row<dev::RtOps, KB, NV>(gt, key, spec, sp, kraw, cv, peel_min, ins);
         }
         if (use_lds) {
             __syncthreads();
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_agg_kernel(const BQuery* __
                 }
                 unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key_signed), ins);
                 if (!g) continue;
-                dev::merge_states(spec, [&](int a) { return g + 1 + a; },
+                dev::merge_states<dev::RtOps>(spec, [&](int a) { return g + 1 + a; },
                                   [&](int a) { return tab.st[(size_t)a * tab.T + s]; });
             }
         }

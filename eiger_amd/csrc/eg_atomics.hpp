@@ -9,7 +9,9 @@
 // straight to the memory atomic.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #include "eg_device.hpp"
@@ -39,7 +41,7 @@ constexpr int kPeelTries = 2;
 //               op a lives at base + a * stride
 // Returns true when this lane's row was merged here.  Every lane of the wave
 // must call it with the same nops and min_lanes.
-template <class OperandFn, class BaseFn>
+template <class OPS, class OperandFn, class BaseFn>
 __device__ __forceinline__ bool wave_peel(bool pending, uint64_t tag, const AggSpec& spec, int min_lanes,
                                           OperandFn operand, BaseFn base, int64_t stride)
 {
@@ -57,12 +59,12 @@ __device__ __forceinline__ bool wave_peel(bool pending, uint64_t tag, const AggS
         const bool mine = (m >> lane) & 1ull;
         decltype(base(0ull)) p = nullptr;
         if (lane == leader) p = base(ltag);
-        for (int a = 0; a < spec.nops; ++a) {
-            const int32_t cls = class_of(spec, a);
+        for_each_op<OPS>(spec, [&](auto a) {
+            const int32_t cls = OPS::cls(spec, a);
             uint64_t v = operand(a);
             if (cls != CLS_XCHG) v = wave_reduce(cls, mine ? v : 0ull);
-            if (lane == leader && p) apply_atomic(cls, p + a * stride, v);
-        }
+            if (lane == leader && p) apply_atomic(cls, p + (int)a * stride, v);
+        });
         done = done || mine;
     }
     return done;

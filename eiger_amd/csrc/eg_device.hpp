@@ -3,8 +3,13 @@
 // emission of one finished group into the output columns.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
+
+#include <type_traits>
+#include <utility>
 
 #include "eg_spec.hpp"
 
@@ -146,18 +151,79 @@ __device__ __forceinline__ uint64_t canon_c(uint64_t raw)
     else return raw;
 }
 
+// ---- op lists ----------------------------------------------------------------
+// Kernels read their aggregation ops through an op-list policy:
+//   RtOps      the AggSpec's ops, read from the kernel arguments: one
+//              compiled kernel serves every request list;
+//   FixOps<W>  the same ops as template constants (W = code | (col + 1) << 4,
+//              in class order), so a row's work folds to straight-line code:
+//              exactly these operand transforms and atomics.
+// Both read the value columns from the AggSpec and give identical results.
+struct RtOps {
+    static constexpr bool kFixed = false;
+    __device__ static int n(const AggSpec& s) { return s.nops; }
+    __device__ static int col(const AggSpec& s, int a) { return s.ops[a].col; }
+    __device__ static uint64_t xmask(const AggSpec& s, int a) { return s.ops[a].xmask; }
+    __device__ static uint64_t fmask(const AggSpec& s, int a) { return s.ops[a].fmask; }
+    // class of op a from the class-ordered list
+    __device__ static int cls(const AggSpec& s, int a)
+    {
+        const int e0 = s.ncls[0], e1 = e0 + s.ncls[1], e2 = e1 + s.ncls[2];
+        return a < e0 ? CLS_ADD_U64 : a < e1 ? CLS_ADD_F64 : a < e2 ? CLS_MAX_U64 : CLS_XCHG;
+    }
+};
+
+template <uint32_t... W>
+struct FixOps {
+    static constexpr bool kFixed = true;
+    static constexpr int kN = sizeof...(W);
+    static_assert(kN >= 1 && kN <= kMaxOps, "op count");
+    __device__ static constexpr int word(int a)
+    {
+        constexpr uint32_t w[kN] = {W...};
+        return (int)w[a];
+    }
+    __device__ static constexpr int n(const AggSpec&) { return kN; }
+    __device__ static constexpr int col(const AggSpec&, int a) { return ((word(a) >> 4) & 15) - 1; }
+    __device__ static constexpr uint64_t xmask(const AggSpec&, int a) { return op_xmask(word(a) & 15); }
+    __device__ static constexpr uint64_t fmask(const AggSpec&, int a) { return op_fmask(word(a) & 15); }
+    __device__ static constexpr int cls(const AggSpec&, int a) { return op_class(word(a) & 15); }
+};
+
+template <int... I, class F>
+__device__ __forceinline__ void static_for_seq(std::integer_sequence<int, I...>, F&& f)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
+// f(a) for every op a (unrolled with constant a for a fixed op list).
+template <class OPS, class F>
+__device__ __forceinline__ void for_each_op(const AggSpec& s, F&& f)
+{
+    if constexpr (OPS::kFixed) static_for_seq(std::make_integer_sequence<int, OPS::kN>{}, f);
+    else
+        for (int a = 0; a < s.nops; ++a) f(a);
+}
+
+// Operand of op a for a row with canonical column values cv (see OpDesc).
+template <class OPS, int NV>
+__device__ __forceinline__ uint64_t op_operand(const AggSpec& s, int a, const uint64_t (&cv)[NV])
+{
+    const int c = OPS::col(s, a);
+    uint64_t v = 1ull;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        if (k == c) v = cv[k];
+    return v ^ OPS::xmask(s, a) ^ ((uint64_t)((int64_t)v >> 63) & OPS::fmask(s, a));
+}
+
 // Operand of op `od` for canonical value v (see OpDesc).
 __device__ __forceinline__ uint64_t operand(const OpDesc& od, uint64_t v)
 {
     return v ^ od.xmask ^ ((uint64_t)((int64_t)v >> 63) & od.fmask);
 }
 
-// Class of op a from the class-ordered op list.
-__device__ __forceinline__ int32_t class_of(const AggSpec& spec, int a)
-{
-    const int e0 = spec.ncls[0], e1 = e0 + spec.ncls[1], e2 = e1 + spec.ncls[2];
-    return a < e0 ? CLS_ADD_U64 : a < e1 ? CLS_ADD_F64 : a < e2 ? CLS_MAX_U64 : CLS_XCHG;
-}
+__device__ __forceinline__ int32_t class_of(const AggSpec& spec, int a) { return RtOps::cls(spec, a); }
 
 // ---- state updates ---------------------------------------------------------
 // Merge operand `v` into the 64-bit state at `p` (LDS or global; the compiler
@@ -186,89 +252,39 @@ __device__ __forceinline__ uint64_t combine(int32_t cls, uint64_t a, uint64_t b)
     }
 }
 
-// Merges the row (canonical column values cv) into the states at
-// ptr(a) for every op, one tight loop per atomic class.
-template <int NV, class PtrFn>
+// Merges the row (canonical column values cv) into the states at ptr(a) for
+// every op: one tight loop per atomic class for the runtime list, straight
+// code for a fixed one.
+template <class OPS, int NV, class PtrFn>
 __device__ __forceinline__ void apply_ops(const AggSpec& spec, const uint64_t (&cv)[NV], PtrFn ptr)
 {
-    auto val = [&](int a) -> uint64_t {
-        const OpDesc& od = spec.ops[a];
-        uint64_t v = 1ull;
-#pragma unroll
-        for (int c = 0; c < NV; ++c)
-            if (c == od.col) v = cv[c];
-        return operand(od, v);
-    };
-    const int e0 = spec.ncls[0], e1 = e0 + spec.ncls[1], e2 = e1 + spec.ncls[2];
-    int a = 0;
-    for (; a < e0; ++a) atomicAdd((unsigned long long*)ptr(a), (unsigned long long)val(a));
-    for (; a < e1; ++a) atomicAdd((double*)ptr(a), __longlong_as_double((long long)val(a)));
-    for (; a < e2; ++a) atomicMax((unsigned long long*)ptr(a), (unsigned long long)val(a));
-    for (; a < spec.nops; ++a) atomicExch((unsigned long long*)ptr(a), (unsigned long long)val(a));
-}
-
-// Packed operand transform (XForm) on 32-bit halves; kind is wave-uniform.
-__device__ __forceinline__ uint64_t xform(uint32_t kind, uint64_t v)
-{
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-    switch (kind) {
-    case XF_MAXI: hi ^= 0x80000000u; break;
-    case XF_MINI: lo = ~lo; hi ^= 0x7fffffffu; break;
-    case XF_MINU: lo = ~lo; hi = ~hi; break;
-    case XF_MAXF: {
-        const uint32_t m = (uint32_t)((int32_t)hi >> 31);
-        lo ^= m;
-        hi ^= (m & 0x7fffffffu) ^ 0x80000000u;
-        break;
-    }
-    case XF_MINF: {
-        const uint32_t m = (uint32_t)((int32_t)hi >> 31);
-        lo ^= ~m;
-        hi ^= ~((m & 0x7fffffffu) ^ 0x80000000u);
-        break;
-    }
-    default: break;
-    }
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Hot-loop variant for up to MAXOPS ops: the op loop is unrolled and each
-// op is one packed 32-bit word (stays in a scalar register), so the row costs
-// a column select, the transform and one atomic per op.  State of op a is at
-// base + a * stride.
-template <int NV, int MAXOPS, class P>
-__device__ __forceinline__ void apply_packed(const AggSpec& spec, const uint64_t (&cv)[NV], P* base, int64_t stride)
-{
-#pragma unroll
-    for (int a = 0; a < MAXOPS; ++a) {
-        if (a >= spec.nops) break;
-        const uint32_t pk = spec.ops[a].packed;
-        const uint32_t colp1 = (pk >> 4) & 15u;
-        uint64_t v = 1ull;
-#pragma unroll
-        for (int c = 0; c < NV; ++c)
-            if ((uint32_t)(c + 1) == colp1) v = cv[c];
-        v = xform(pk & 15u, v);
-        P* p = base + a * stride;
-        switch ((pk >> 8) & 3u) {
-        case CLS_ADD_U64: atomicAdd((unsigned long long*)p, (unsigned long long)v); break;
-        case CLS_ADD_F64: atomicAdd((double*)p, __longlong_as_double((long long)v)); break;
-        case CLS_MAX_U64: atomicMax((unsigned long long*)p, (unsigned long long)v); break;
-        default: atomicExch((unsigned long long*)p, (unsigned long long)v); break;
-        }
+    if constexpr (OPS::kFixed) {
+        for_each_op<OPS>(spec, [&](auto a) { apply_atomic(OPS::cls(spec, a), ptr(a), op_operand<OPS, NV>(spec, a, cv)); });
+    } else {
+        auto val = [&](int a) { return op_operand<OPS, NV>(spec, a, cv); };
+        const int e0 = spec.ncls[0], e1 = e0 + spec.ncls[1], e2 = e1 + spec.ncls[2];
+        int a = 0;
+        for (; a < e0; ++a) atomicAdd((unsigned long long*)ptr(a), (unsigned long long)val(a));
+        for (; a < e1; ++a) atomicAdd((double*)ptr(a), __longlong_as_double((long long)val(a)));
+        for (; a < e2; ++a) atomicMax((unsigned long long*)ptr(a), (unsigned long long)val(a));
+        for (; a < spec.nops; ++a) atomicExch((unsigned long long*)ptr(a), (unsigned long long)val(a));
     }
 }
 
 // Same, merging 64-bit partial states st(a) (already operands).
-template <class PtrFn, class StFn>
+template <class OPS, class PtrFn, class StFn>
 __device__ __forceinline__ void merge_states(const AggSpec& spec, PtrFn ptr, StFn st)
 {
-    const int e0 = spec.ncls[0], e1 = e0 + spec.ncls[1], e2 = e1 + spec.ncls[2];
-    int a = 0;
-    for (; a < e0; ++a) atomicAdd((unsigned long long*)ptr(a), (unsigned long long)st(a));
-    for (; a < e1; ++a) atomicAdd((double*)ptr(a), __longlong_as_double((long long)st(a)));
-    for (; a < e2; ++a) atomicMax((unsigned long long*)ptr(a), (unsigned long long)st(a));
-    for (; a < spec.nops; ++a) atomicExch((unsigned long long*)ptr(a), (unsigned long long)st(a));
+    if constexpr (OPS::kFixed) {
+        for_each_op<OPS>(spec, [&](auto a) { apply_atomic(OPS::cls(spec, a), ptr(a), (uint64_t)st(a)); });
+    } else {
+        const int e0 = spec.ncls[0], e1 = e0 + spec.ncls[1], e2 = e1 + spec.ncls[2];
+        int a = 0;
+        for (; a < e0; ++a) atomicAdd((unsigned long long*)ptr(a), (unsigned long long)st(a));
+        for (; a < e1; ++a) atomicAdd((double*)ptr(a), __longlong_as_double((long long)st(a)));
+        for (; a < e2; ++a) atomicMax((unsigned long long*)ptr(a), (unsigned long long)st(a));
+        for (; a < spec.nops; ++a) atomicExch((unsigned long long*)ptr(a), (unsigned long long)st(a));
+    }
 }
 
 // ---- output ----------------------------------------------------------------

@@ -5,11 +5,15 @@
 // of an empty slot only costs one extra CAS.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #include "eg_device.hpp"
+#ifndef __HIPCC_RTC__
 #include "eg_internal.hpp"
+#endif
 
 namespace eg {
 
@@ -65,6 +69,7 @@ __device__ __forceinline__ void gt_count_inserts(const GTable& gt, uint32_t ins)
     if (dev::lane_id() == 0 && ins != 0u) atomicAdd(&gt.meta[0], (unsigned long long)ins);
 }
 
+#ifndef __HIPCC_RTC__  // host side
 int table_words_per_slot(int nops);
 
 // Owns a zeroed HBM table of `slots` (power of two) slots.
@@ -87,5 +92,6 @@ struct HashLayout {
 };
 
 HashLayout plan_hash_layout(int nops, int key_bytes, int64_t group_bound, const HashTuning& t);
+#endif
 
 }  // namespace eg

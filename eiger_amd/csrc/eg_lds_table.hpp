@@ -5,14 +5,15 @@
 // combining of hot groups).
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 
 #include <type_traits>
 
 #include "eg_atomics.hpp"
 #include "eg_device.hpp"
 #include "eg_hash_table.hpp"
-#include "eg_internal.hpp"
 
 namespace eg {
 namespace lds {
@@ -42,16 +43,6 @@ __device__ __forceinline__ void vload4(const void* p, int64_t r, uint64_t (&o)[k
 __device__ __forceinline__ uint64_t pick4(const uint64_t (&a)[kRows], int i)
 {
     return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3];
-}
-
-template <int NV>
-__device__ __forceinline__ uint64_t pick_col(const uint64_t (&v)[NV], int c)
-{
-    uint64_t x = 0;
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-        if (k == c) x = v[k];
-    return x;
 }
 
 // Value-column layouts compiled into the kernel.  GEN: any widths, read with
@@ -234,15 +225,8 @@ __device__ __forceinline__ void canon_row(const AggSpec& spec, const uint64_t (&
     }
 }
 
-template <int NV>
-__device__ __forceinline__ uint64_t operand_of(const AggSpec& spec, int a, const uint64_t (&cv)[NV])
-{
-    const OpDesc& od = spec.ops[a];
-    return dev::operand(od, od.col < 0 ? 1ull : pick_col<NV>(cv, od.col));
-}
-
 // One row into the LDS table; returns true when it needs the HBM table.
-template <int KB, int NV>
+template <class OPS, int KB, int NV>
 __device__ __forceinline__ bool lds_row(LdsTable<KB>& tab, bool use_lds, int r_mask, int lane, const KeyDesc& key,
                                         const AggSpec& spec, uint64_t kraw, const uint64_t (&cv)[NV],
                                         bool valid, int probe_limit, int peel_min)
@@ -268,23 +252,24 @@ __device__ __forceinline__ bool lds_row(LdsTable<KB>& tab, bool use_lds, int r_m
     }
     bool done = false;
     if (use_lds && r_mask == 0 && peel_min <= 64) {
-        done = dev::wave_peel(slot >= 0, (uint64_t)(int64_t)slot, spec, peel_min,
-                              [&](int a) { return operand_of<NV>(spec, a, cv); },
+        done = dev::wave_peel<OPS>(slot >= 0, (uint64_t)(int64_t)slot, spec, peel_min,
+                                   [&](int a) { return dev::op_operand<OPS, NV>(spec, a, cv); },
                               [&](uint64_t s) { return &tab.st[(int64_t)s]; }, (int64_t)tab.T);
     }
-    if (slot >= 0 && !done) dev::apply_ops<NV>(spec, cv, [&](int a) { return &tab.st[(size_t)a * tab.T + slot]; });
+    if (slot >= 0 && !done) dev::apply_ops<OPS, NV>(spec, cv, [&](int a) { return &tab.st[(size_t)a * tab.T + slot]; });
     return valid && slot < 0;
 }
 
 // One row into the HBM table (called by every lane of the wave).
-template <int KB, int NV>
+template <class OPS, int KB, int NV>
 __device__ __forceinline__ void hbm_row(const GTable& gt, const KeyDesc& key, const AggSpec& spec, bool pending,
                                         uint64_t kraw, const uint64_t (&cv)[NV], int peel_min, uint32_t& ins)
 {
     const uint64_t e64 = dev::enc64(dev::widen_key(kraw, KB, key.is_signed));
     bool done = false;
     if (peel_min <= 64)
-        done = dev::wave_peel(pending, e64, spec, peel_min, [&](int a) { return operand_of<NV>(spec, a, cv); },
+        done = dev::wave_peel<OPS>(pending, e64, spec, peel_min,
+                                   [&](int a) { return dev::op_operand<OPS, NV>(spec, a, cv); },
                               [&](uint64_t t) {
                                   unsigned long long* g = gt_slot(gt, t, ins);
                                   return g ? g + 1 : (unsigned long long*)nullptr;
@@ -292,7 +277,7 @@ __device__ __forceinline__ void hbm_row(const GTable& gt, const KeyDesc& key, co
                               (int64_t)1);
     if (pending && !done) {
         unsigned long long* g = gt_slot(gt, e64, ins);
-        if (g) dev::apply_ops<NV>(spec, cv, [&](int a) { return g + 1 + a; });
+        if (g) dev::apply_ops<OPS, NV>(spec, cv, [&](int a) { return g + 1 + a; });
     }
 }
 

@@ -39,14 +39,12 @@ enum OpCode : int32_t {
 // Update classes (the atomic that merges an operand into a state).
 enum OpClass : int32_t { CLS_ADD_U64 = 0, CLS_ADD_F64 = 1, CLS_MAX_U64 = 2, CLS_XCHG = 3 };
 
-__host__ __device__ inline int32_t op_class(int32_t code)
+__host__ __device__ constexpr int32_t op_class(int32_t code)
 {
-    switch (code) {
-    case OPC_COUNT: case OPC_SUM_I: case OPC_SUM_U: return CLS_ADD_U64;
-    case OPC_SUM_F: return CLS_ADD_F64;
-    case OPC_ANY: return CLS_XCHG;
-    default: return CLS_MAX_U64;
-    }
+    return (code == OPC_COUNT || code == OPC_SUM_I || code == OPC_SUM_U) ? CLS_ADD_U64
+         : code == OPC_SUM_F ? CLS_ADD_F64
+         : code == OPC_ANY ? CLS_XCHG
+         : CLS_MAX_U64;
 }
 
 // How a column's raw bits become its canonical 64-bit value: integers
@@ -69,23 +67,24 @@ struct ColDesc {
 //   MIN(int)                    xmask ~2^63,  fmask 0
 //   MAX(float)                  xmask 2^63,   fmask ~2^63   (order-preserving bits)
 //   MIN(float)                  xmask ~2^63,  fmask ~2^63
+__host__ __device__ constexpr uint64_t op_xmask(int32_t code)
+{
+    return code == OPC_MIN_U ? ~0ull
+         : (code == OPC_MAX_I || code == OPC_MAX_F) ? 0x8000000000000000ull
+         : (code == OPC_MIN_I || code == OPC_MIN_F) ? 0x7fffffffffffffffull
+         : 0ull;
+}
+
+__host__ __device__ constexpr uint64_t op_fmask(int32_t code)
+{
+    return (code == OPC_MAX_F || code == OPC_MIN_F) ? 0x7fffffffffffffffull : 0ull;
+}
+
 struct OpDesc {
     int32_t code;       // OpCode
     int32_t col;        // index into AggSpec::cols, -1 for COUNT
-    uint64_t xmask;
-    uint64_t fmask;
-    uint32_t packed;    // hot-loop form: xform | (col + 1) << 4 | class << 8
-    uint32_t pad;
-};
-
-// Operand transforms of the packed form (same results as xmask/fmask).
-enum XForm : uint32_t {
-    XF_NONE = 0,   // SUM, COUNT, ANY, MAX(uint)
-    XF_MAXI = 1,   // flip bit 63
-    XF_MINI = 2,   // xor ~2^63
-    XF_MINU = 3,   // bitwise not
-    XF_MAXF = 4,   // order-preserving double bits
-    XF_MINF = 5    // complement of the above
+    uint64_t xmask;     // op_xmask(code)
+    uint64_t fmask;     // op_fmask(code)
 };
 
 // Ops are ordered by class: ncls[0] add-u64, then ncls[1] add-f64, then

@@ -28,148 +28,26 @@
 #include <algorithm>
 #include <cstring>
 
+#include "eg_agg_kernels.hpp"
 #include "eg_atomics.hpp"
 #include "eg_device.hpp"
 #include "eg_hash_table.hpp"
 #include "eg_internal.hpp"
+#include "eg_jit.hpp"
 #include "eg_lds_table.hpp"
 
 namespace eg {
 namespace {
 
 using namespace lds;
-
-// Wider value sets need more registers per lane: cap the workgroup at 512.
-template <int NV>
-constexpr int max_block() { return NV <= 2 ? 1024 : 512; }
-
-// Merges the replicas of the block's LDS table and adds every group to the
-// HBM table.
-template <int KB>
-__device__ __forceinline__ void flush_lds(LdsTable<KB>& tab, const GTable& gt, const KeyDesc& key, const AggSpec& spec,
-                          int rep_log2, int r_mask, uint32_t& ins)
-{
-    using KT = typename LdsKey<KB>::T;
-    __syncthreads();
-    // merge replicas 1..R-1 into replica 0 (LDS atomics)
-    const int R = 1 << rep_log2;
-    if (R > 1) {
-        for (int idx = threadIdx.x; idx < tab.T; idx += blockDim.x) {
-            const int r = idx & r_mask;
-            if (r == 0) continue;
-            const int s = idx >> rep_log2;
-            int dst;
-            KT kw = 0;
-            if (s == tab.cr) {
-                if (tab.side[r] == 0ull) continue;
-                dst = tab.find((KT)0, 0, 1);
-            } else {
-                kw = tab.keys[idx];
-                if (kw == 0) continue;
-                dst = tab.find(kw, 0, tab.cr);
-            }
-            auto src = [&](int a) { return tab.st[(size_t)a * tab.T + idx]; };
-            if (dst >= 0) {
-                dev::merge_states(spec, [&](int a) { return &tab.st[(size_t)a * tab.T + dst]; }, src);
-            } else {
-                // replica 0 is full: merge this slot straight into HBM
-                unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed), ins);
-                if (g) dev::merge_states(spec, [&](int a) { return g + 1 + a; }, src);
-            }
-        }
-        __syncthreads();
-    }
-    // flush replica 0 into the HBM table
-    for (int s = threadIdx.x; s <= tab.cr; s += blockDim.x) {
-        const int idx = s << rep_log2;
-        KT kw;
-        if (s == tab.cr) {
-            if (tab.side[0] == 0ull) continue;
-            kw = 0;
-        } else {
-            kw = tab.keys[idx];
-            if (kw == 0) continue;
-        }
-        unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed), ins);
-        if (!g) continue;
-        dev::merge_states(spec, [&](int a) { return g + 1 + a; }, [&](int a) { return tab.st[(size_t)a * tab.T + idx]; });
-    }
-}
+using agg::max_block;
 
 template <int KB, class VL>
 __global__ __launch_bounds__(max_block<VL::NV>()) void hash_agg_kernel(KeyDesc key, int64_t n, AggSpec spec,
                                                                         GTable gt, int cr_log2, int rep_log2,
                                                                         int probe_limit, int peel_min)
 {
-    constexpr int NV = VL::NV > 0 ? VL::NV : 1;
-    using KT = typename LdsKey<KB>::T;
-    extern __shared__ unsigned long long smem[];
-    const bool use_lds = cr_log2 >= 0;
-    const int nops = spec.nops;
-    const int lane = dev::lane_id();
-    LdsTable<KB> tab(smem, nops, use_lds ? cr_log2 : 0, use_lds ? rep_log2 : 0);
-    if (use_lds) {
-        const int words = LdsTable<KB>::words(nops, cr_log2, rep_log2);
-        for (int i = threadIdx.x; i < words; i += blockDim.x) smem[i] = 0ull;
-        __syncthreads();
-    }
-    const int r_mask = (1 << (use_lds ? rep_log2 : 0)) - 1;
-    uint32_t ins = 0;  // HBM-table insertions by this lane
-
-    // ---- full tiles: vector loads, next tile in flight while this one runs ----
-    const int64_t tile = (int64_t)blockDim.x * kRows;
-    const int64_t full = n / tile;
-    int64_t t = blockIdx.x;
-    uint64_t kcur[kRows], vcur[NV][kRows];
-    if (t < full) load_tile<VL>(key, KB, spec, t * tile + (int64_t)threadIdx.x * kRows, kcur, vcur);
-    for (; t < full; t += gridDim.x) {
-        uint64_t knxt[kRows], vnxt[NV][kRows];
-        const int64_t tn = t + gridDim.x;
-        if (tn < full) load_tile<VL>(key, KB, spec, tn * tile + (int64_t)threadIdx.x * kRows, knxt, vnxt);
-        uint32_t spill = 0;
-#pragma unroll
-        for (int i = 0; i < kRows; ++i) {
-            uint64_t raw[NV], cv[NV];
-#pragma unroll
-            for (int c = 0; c < NV; ++c) raw[c] = vcur[c][i];
-            canon_row<VL, NV>(spec, raw, cv);
-            if (lds_row<KB, NV>(tab, use_lds, r_mask, lane, key, spec, kcur[i], cv, true, probe_limit, peel_min))
-                spill |= 1u << i;
-        }
-        if (__ballot(spill != 0) != 0ull) {
-#pragma unroll 1
-            for (int i = 0; i < kRows; ++i) {
-                const bool pend = (spill >> i) & 1u;
-                if (__ballot(pend) == 0ull) continue;
-                uint64_t raw[NV], cv[NV];
-#pragma unroll
-                for (int c = 0; c < NV; ++c) raw[c] = pick4(vcur[c], i);
-                canon_row<VL, NV>(spec, raw, cv);
-                hbm_row<KB, NV>(gt, key, spec, pend, pick4(kcur, i), cv, peel_min, ins);
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < kRows; ++i) {
-            kcur[i] = knxt[i];
-#pragma unroll
-            for (int c = 0; c < NV; ++c) vcur[c][i] = vnxt[c][i];
-        }
-    }
-    // ---- tail rows: one row per lane, typed loads ----
-    for (int64_t b = full * tile + (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = b + threadIdx.x;
-        const bool valid = r < n;
-        const uint64_t kraw = valid ? dev::load_raw(key.data, KB, r) : 0ull;
-        uint64_t raw[NV], cv[NV];
-#pragma unroll
-        for (int c = 0; c < NV; ++c)
-            raw[c] = (valid && c < spec.nvcols) ? dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, r) : 0ull;
-        canon_row<VL, NV>(spec, raw, cv);
-        const bool sp = lds_row<KB, NV>(tab, use_lds, r_mask, lane, key, spec, kraw, cv, valid, probe_limit, peel_min);
-        if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kraw, cv, peel_min, ins);
-    }
-    if (use_lds) flush_lds(tab, gt, key, spec, rep_log2, r_mask, ins);
-    gt_count_inserts(gt, ins);
+    agg::hash_agg<KB, VL, dev::RtOps>(key, n, spec, gt, cr_log2, rep_log2, probe_limit, peel_min);
 }
 
 __global__ void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter)
@@ -203,15 +81,26 @@ void launch_agg(const DeviceInfo& di, const HashTuning& t, const KeyDesc& key, i
     auto kern = hash_agg_kernel<KB, VL>;
     enable_dynamic_lds((const void*)kern);
     const int block = std::min(t.block, max_block<VL::NV>());
-    int per_cu = t.blocks_per_cu;
-    if (per_cu <= 0) {
-        EG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, L.lds_bytes));
-        per_cu = std::max(per_cu, 1);
-    }
     const int64_t tile = (int64_t)block * kRows;
     const int64_t want = (n + tile - 1) / tile;
-    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)di.num_cus * per_cu));
+    auto grid_for = [&](int per_cu) {
+        return std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)di.num_cus * std::max(per_cu, 1)));
+    };
     KernelTimer timer("hash_agg", s);
+    // the kernel with this request's ops compiled in, once it is available
+    if (hipFunction_t f = jit::lookup(jit::Kind::HASH_AGG, KB, jit::vlayout_name<VL>(), spec, n)) {
+        const int64_t grid = grid_for(t.blocks_per_cu > 0 ? t.blocks_per_cu : jit::occupancy(f, block, L.lds_bytes));
+        KeyDesc a_key = key;
+        int64_t a_n = n;
+        AggSpec a_spec = spec;
+        GTable a_gt = gt;
+        int a_cr = L.cr_log2, a_rep = L.rep_log2, a_probe = L.probe_limit, a_peel = L.peel_min;
+        void* args[] = {&a_key, &a_n, &a_spec, &a_gt, &a_cr, &a_rep, &a_probe, &a_peel};
+        if (jit::launch(f, (unsigned)grid, (unsigned)block, L.lds_bytes, s, args)) return;
+    }
+    int per_cu = t.blocks_per_cu;
+    if (per_cu <= 0) EG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, L.lds_bytes));
+    const int64_t grid = grid_for(per_cu);
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), L.lds_bytes, s, key, n, spec, gt, L.cr_log2,
                        L.rep_log2, L.probe_limit, L.peel_min);
     EG_LAUNCH_CHECK();

@@ -28,10 +28,12 @@
 #include <memory>
 #include <vector>
 
+#include "eg_agg_kernels.hpp"
 #include "eg_atomics.hpp"
 #include "eg_device.hpp"
 #include "eg_hash_table.hpp"
 #include "eg_internal.hpp"
+#include "eg_jit.hpp"
 #include "eg_lds_table.hpp"
 #include "eg_radix.hpp"
 
@@ -39,8 +41,6 @@ namespace eg {
 namespace {
 
 using namespace lds;
-
-constexpr int kPBMax = 1024;  // largest workgroup of the unit kernel
 
 // Digit width of one partitioning pass.  Fan-out 64 keeps the rows of one
 // digit in runs long enough for full-line stores (measured: a 6-bit pass on
@@ -68,12 +68,18 @@ PartTune part_tune()
     return t;
 }
 
-struct Unit {
-    int64_t start;
-    int64_t end;
-    int32_t sliced;  // the partition spans several units
-    int32_t pad;
-};
+using agg::kPBMax;
+using agg::Unit;
+
+template <int KB, class VL>
+__global__ __launch_bounds__(kPBMax) void part_agg_kernel(KeyDesc key, AggSpec spec, const int64_t* perm,
+                                                       const Unit* units, GTable gt, OutSpec os,
+                                                       unsigned long long* counter, int64_t capacity,
+                                                       int cr_log2, int probe_limit, int peel_min)
+{
+    agg::part_agg<KB, VL, dev::RtOps>(key, spec, perm, units, gt, os, counter, capacity, cr_log2, probe_limit,
+                                      peel_min);
+}
 
 template <int KB>
 __device__ __forceinline__ uint32_t pid_of(uint64_t raw, uint32_t mask)
@@ -92,76 +98,6 @@ __global__ void part_bounds_kernel(const void* keys, int64_t n, uint32_t mask, i
         if (i == n - 1)
             for (int64_t q = p + 1; q <= (int64_t)mask + 1; ++q) offs[q] = n;
     }
-}
-
-template <int KB, class VL>
-__global__ __launch_bounds__(kPBMax) void part_agg_kernel(KeyDesc key, AggSpec spec, const int64_t* perm,
-                                                       const Unit* units, GTable gt, OutSpec os,
-                                                       unsigned long long* counter, int64_t capacity,
-                                                       int cr_log2, int probe_limit, int peel_min)
-{
-    constexpr int NV = VL::NV > 0 ? VL::NV : 1;
-    constexpr int J = 4;  // rows per lane per step: independent loads in flight
-    using KT = typename LdsKey<KB>::T;
-    extern __shared__ unsigned long long smem[];
-    __shared__ unsigned long long sh[17];
-    const int nops = spec.nops;
-    const int lane = dev::lane_id();
-    LdsTable<KB> tab(smem, nops, cr_log2, 0);
-    const int words = LdsTable<KB>::words(nops, cr_log2, 0);
-    for (int i = threadIdx.x; i < words; i += blockDim.x) smem[i] = 0ull;
-    __syncthreads();
-
-    const Unit u = units[blockIdx.x];
-    uint32_t ins = 0;  // HBM-table insertions by this lane
-    for (int64_t b = u.start; b < u.end; b += (int64_t)J * blockDim.x) {
-        uint64_t kr[J], raw[J][NV];
-        bool valid[J];
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            const int64_t r = b + (int64_t)j * blockDim.x + threadIdx.x;
-            valid[j] = r < u.end;
-            const int64_t src = (perm && valid[j]) ? perm[r] : r;
-            kr[j] = valid[j] ? (KB == 4 ? (uint64_t)((const uint32_t*)key.data)[r] : ((const uint64_t*)key.data)[r]) : 0ull;
-            load_row<VL>(spec, src, valid[j], raw[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            uint64_t cv[NV];
-            canon_row<VL, NV>(spec, raw[j], cv);
-            const bool sp = lds_row<KB, NV>(tab, true, 0, lane, key, spec, kr[j], cv, valid[j], probe_limit, peel_min);
-            if (__ballot(sp) != 0ull) hbm_row<KB, NV>(gt, key, spec, sp, kr[j], cv, peel_min, ins);
-        }
-    }
-    __syncthreads();
-    // finished groups of this unit
-    for (int base = 0; base <= tab.cr; base += blockDim.x) {
-        const int s = base + threadIdx.x;
-        bool occ = false;
-        KT kw = 0;
-        if (s < tab.cr) {
-            kw = tab.keys[s];
-            occ = kw != 0;
-        } else if (s == tab.cr) {
-            occ = tab.side[0] != 0ull;
-        }
-        const uint64_t enc = lds_to_enc64<KB>(kw, key.is_signed);
-        if (u.sliced) {
-            if (occ) {
-                unsigned long long* g = gt_slot(gt, enc, ins);
-                if (g)
-                    dev::merge_states(spec, [&](int a) { return g + 1 + a; },
-                                      [&](int a) { return tab.st[(size_t)a * tab.T + s]; });
-            }
-        } else {
-            const int64_t idx = dev::block_claim(counter, occ, sh);
-            if (occ) {
-                if (idx < capacity) dev::emit_group(os, idx, enc, tab.st + s, (int64_t)tab.T);
-                else atomicExch(&gt.meta[2], 1ull);  // output capacity exceeded: retry larger
-            }
-        }
-    }
-    gt_count_inserts(gt, ins);
 }
 
 // Appends the HBM table's groups after the directly written ones.
@@ -198,14 +134,31 @@ int ceil_log2(int64_t x)
 template <int KB>
 void launch_part_agg(const KeyDesc& key, const AggSpec& spec, const int64_t* perm, const Unit* units, int nunits,
                      const GTable& gt, const OutSpec& os, unsigned long long* counter, int64_t capacity, int cr_log2,
-                     size_t lds, int block, hipStream_t s)
+                     size_t lds, int block, int64_t rows, hipStream_t s)
 {
     with_vlayout(spec, true, [&](auto vl) {
-        auto kern = part_agg_kernel<KB, decltype(vl)>;
+        using VL = decltype(vl);
+        auto kern = part_agg_kernel<KB, VL>;
         if (enable_dynamic_lds((const void*)kern) < lds) raise(EG_ERR_UNSUPPORTED, "partition LDS table too large");
         KernelTimer timer("partition_agg", s);
+        int probe_limit = 64, peel_min = 8;
+        // the kernel with this request's ops compiled in, once it is available
+        if (hipFunction_t f = jit::lookup(jit::Kind::PART_AGG, KB, jit::vlayout_name<VL>(), spec, rows)) {
+            KeyDesc a_key = key;
+            AggSpec a_spec = spec;
+            const int64_t* a_perm = perm;
+            const Unit* a_units = units;
+            GTable a_gt = gt;
+            OutSpec a_os = os;
+            unsigned long long* a_counter = counter;
+            int64_t a_cap = capacity;
+            int a_cr = cr_log2;
+            void* args[] = {&a_key, &a_spec, &a_perm, &a_units, &a_gt, &a_os, &a_counter, &a_cap, &a_cr,
+                            &probe_limit, &peel_min};
+            if (jit::launch(f, (unsigned)nunits, (unsigned)block, lds, s, args)) return;
+        }
         hipLaunchKernelGGL(kern, dim3(nunits), dim3(block), lds, s, key, spec, perm, units, gt, os, counter, capacity,
-                           cr_log2, 64, 8);
+                           cr_log2, probe_limit, peel_min);
         EG_LAUNCH_CHECK();
     });
 }
@@ -379,10 +332,10 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
         fill_zero(counter.get(), 8, s);
         if (kb == 4)
             launch_part_agg<4>(pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
-                               counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, s);
+                               counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, n, s);
         else
             launch_part_agg<8>(pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
-                               counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, s);
+                               counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, n, s);
         int64_t meta[3];
         ht.read_meta(meta, s);
         const bool table_full = meta[2] != 0;

@@ -461,6 +461,40 @@ def probe_copy(nbytes: int, iters: int = 10, stream=None) -> float:
     return ms.value
 
 
+# ---------------------------------------------------------------- run-time specialisation
+class jit_mode(IntEnum):
+    OFF = 0
+    ASYNC = 1
+    SYNC = 2
+
+
+def jit_configure(mode: Optional[int] = None, min_rows: Optional[int] = None):
+    """Mode of the run-time kernel specialisation (OFF / ASYNC / SYNC) and the
+    smallest input it is used for; None keeps the current value."""
+    _check(lib().eg_jit_configure(-1 if mode is None else int(mode), -1 if min_rows is None else int(min_rows)))
+
+
+def jit_wait():
+    """Blocks until no specialised kernel is being compiled."""
+    _check(lib().eg_jit_wait())
+
+
+def jit_stats() -> dict:
+    out = (C.c_int64 * 6)()
+    _check(lib().eg_jit_stats(out))
+    return {"compiled": out[0], "failed": out[1], "pending": out[2], "launches": out[3],
+            "mode": out[4], "min_rows": out[5]}
+
+
+def jit_selftest(arch: str = "gfx950") -> None:
+    """Compiles a fixed set of specialised kernels for `arch` (no device
+    needed); raises with the compiler log on failure."""
+    log = C.create_string_buffer(1 << 16)
+    st = lib().eg_jit_selftest(arch.encode(), log, len(log))
+    if st != EG_OK:
+        raise GamaxError(st, lib().eg_last_error().decode() + "\n" + log.value.decode(errors="replace"))
+
+
 # ---------------------------------------------------------------- synthetic data
 def gen_keys_uniform(n: int, distinct: int, seed: int = 42, t: int = type_id.INT32,
                      begin: int = 0, count: Optional[int] = None, stream=None) -> column:

@@ -357,6 +357,29 @@ int eg_kernel_timing_reset(void);
  * since the last reset (synchronises the recorded events). */
 int eg_kernel_timing_query(const char* name, int64_t* launches, double* total_ms);
 
+/* ---- run-time kernel specialisation (no counterpart in the reference: its
+ * CUDA build instantiates one kernel per aggregation functor at compile time,
+ * groupby_hash.cu:77-99; here the op list is a kernel argument and hipRTC
+ * compiles the hash / partition aggregation kernels with the request's ops as
+ * constants) ------------------------------------------------------------- */
+enum { EG_JIT_OFF = 0, EG_JIT_ASYNC = 1, EG_JIT_SYNC = 2 };
+/* mode: EG_JIT_ASYNC (default; compile on a background thread, use the
+ * generic kernel until the specialised one is ready), EG_JIT_SYNC (compile on
+ * first use), EG_JIT_OFF, or -1 to keep the mode.  min_rows: inputs with fewer
+ * rows use the generic kernel (default 2^22; -1 keeps the value).  The
+ * environment variables EG_JIT (off|sync|async) and EG_JIT_MIN_ROWS set the
+ * initial values. */
+int eg_jit_configure(int32_t mode, int64_t min_rows);
+/* Blocks until no compilation is queued or running. */
+int eg_jit_wait(void);
+/* out6 = {kernels compiled, compiles or loads failed, compiles pending,
+ *         specialised launches, mode, min_rows}. */
+int eg_jit_stats(int64_t* out6);
+/* Diagnostic: compiles a fixed set of specialised kernels (hash and partition,
+ * 4- and 8-byte keys, vector and typed-load value layouts) for `arch` (e.g.
+ * "gfx950") without a device; the compiler log of failures goes to log. */
+int eg_jit_selftest(const char* arch, char* log, int64_t log_len);
+
 /* Reference copy kernel for bandwidth calibration: copies `bytes` between two
  * scratch buffers `iters` times with 16-byte vector loads/stores (kernel name
  * "probe_copy"); *ms = mean time per copy.  Mirrors the reference's
