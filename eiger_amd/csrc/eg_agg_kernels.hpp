@@ -165,7 +165,9 @@ struct Unit {
     int32_t pad;
 };
 
-template <int KB, class VL, class OPS>
+// GATHER: values are read through the row-id permutation `perm` (the
+// PARTITION_GFUR layout), else straight from the partitioned columns.
+template <int KB, class VL, class OPS, bool GATHER>
 __device__ __forceinline__ void part_agg(const KeyDesc& key, const AggSpec& spec, const int64_t* perm,
                                          const Unit* units, const GTable& gt, const OutSpec& os,
                                          unsigned long long* counter, int64_t capacity, int cr_log2,
@@ -183,25 +185,49 @@ __device__ __forceinline__ void part_agg(const KeyDesc& key, const AggSpec& spec
     for (int i = threadIdx.x; i < words; i += blockDim.x) smem[i] = 0ull;
     __syncthreads();
 
-    const Unit u = units[blockIdx.x];
+    const Unit u = units[blockIdx.x];  // never empty
     uint32_t ins = 0;  // HBM-table insertions by this lane
-    for (int64_t b = u.start; b < u.end; b += (int64_t)J * blockDim.x) {
-        uint64_t kr[J], raw[J][NV];
-        bool valid[J];
+    // Loads of one step (J rows per lane): all issued before any is used;
+    // rows past the unit's end read its first row instead (in bounds, no
+    // branch) and are masked by `valid`.
+    auto load_step = [&](int64_t b, uint64_t (&kr)[J], uint64_t (&raw)[J][NV], bool (&valid)[J]) {
+        int64_t src[J];
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             const int64_t r = b + (int64_t)j * blockDim.x + threadIdx.x;
             valid[j] = r < u.end;
-            const int64_t src = (perm && valid[j]) ? perm[r] : r;
-            kr[j] = valid[j] ? (KB == 4 ? (uint64_t)((const uint32_t*)key.data)[r] : ((const uint64_t*)key.data)[r]) : 0ull;
-            load_row<VL>(spec, src, valid[j], raw[j]);
+            src[j] = valid[j] ? r : u.start;
+            kr[j] = KB == 4 ? (uint64_t)((const uint32_t*)key.data)[src[j]] : ((const uint64_t*)key.data)[src[j]];
         }
+        if constexpr (GATHER) {
+#pragma unroll
+            for (int j = 0; j < J; ++j) src[j] = perm[src[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) load_row<VL>(spec, src[j], true, raw[j]);
+    };
+    const int64_t step = (int64_t)J * blockDim.x;
+    uint64_t kr[J], raw[J][NV];
+    bool valid[J];
+    load_step(u.start, kr, raw, valid);
+    for (int64_t b = u.start; b < u.end; b += step) {
+        // the next step's rows are in flight while this step aggregates
+        uint64_t kn[J], rn[J][NV];
+        bool vn[J];
+        if (b + step < u.end) load_step(b + step, kn, rn, vn);
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             uint64_t cv[NV];
             canon_row<VL, NV>(spec, raw[j], cv);
             const bool sp = lds_row<OPS, KB, NV>(tab, true, 0, lane, key, spec, kr[j], cv, valid[j], probe_limit, peel_min);
             if (__ballot(sp) != 0ull) hbm_row<OPS, KB, NV>(gt, key, spec, sp, kr[j], cv, peel_min, ins);
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            kr[j] = kn[j];
+            valid[j] = vn[j];
+#pragma unroll
+            for (int c = 0; c < NV; ++c) raw[j][c] = rn[j][c];
         }
     }
     __syncthreads();

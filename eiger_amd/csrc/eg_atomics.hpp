@@ -4,7 +4,7 @@
 // aggregation (atomics/atomics.cuh:10-126, groupby_hash.cu:77-99), which
 // serialises on hot keys.  Here a wavefront first looks for lanes that update
 // the same target: a ballot over the first pending lanes' tags finds them, a
-// cross-lane butterfly reduces their operands, and one lane issues a single
+// cross-lane DPP reduction combines their operands, and one lane issues a single
 // atomic per aggregation state.  Lanes left over (distinct targets) go
 // straight to the memory atomic.
 #pragma once
@@ -19,30 +19,58 @@
 namespace eg {
 namespace dev {
 
-// Butterfly reduction of a 64-bit state over all 64 lanes (every lane of the
-// wave must execute it; lanes outside the group pass the class identity 0).
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xf, 0xf, false);
+    return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
+}
+
+__device__ __forceinline__ uint32_t readlane32(uint32_t v, int lane)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane)
+{
+    return (uint64_t)readlane32((uint32_t)v, lane) | ((uint64_t)readlane32((uint32_t)(v >> 32), lane) << 32);
+}
+
+// Reduction of a 64-bit state over all 64 lanes (every lane of the wave must
+// execute it with the full wave active; lanes outside the group pass the
+// class identity 0).  Within each 16-lane row: DPP quad_perm [1,0,3,2] and
+// [2,3,0,1] (xor 1, xor 2), row_half_mirror (other quad of the 8) and
+// row_mirror (other half of the row) -- every lane then holds its row's
+// total; the four row totals are read out and combined, so the result is
+// wave-uniform.  No LDS traffic, unlike a shuffle butterfly.
 __device__ __forceinline__ uint64_t wave_reduce(int32_t cls, uint64_t v)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        uint64_t o = __shfl_xor((unsigned long long)v, off);
-        v = combine(cls, v, o);
-    }
-    return v;
+    v = combine(cls, v, dpp64<0xb1>(v));
+    v = combine(cls, v, dpp64<0x4e>(v));
+    v = combine(cls, v, dpp64<0x141>(v));
+    v = combine(cls, v, dpp64<0x140>(v));
+    const uint64_t a = combine(cls, readlane64(v, 0), readlane64(v, 16));
+    const uint64_t b = combine(cls, readlane64(v, 32), readlane64(v, 48));
+    return combine(cls, a, b);
 }
 
 constexpr int kPeelTries = 2;
 
+__device__ __forceinline__ uint32_t read_leader(uint32_t tag, int leader) { return readlane32(tag, leader); }
+__device__ __forceinline__ uint64_t read_leader(uint64_t tag, int leader) { return readlane64(tag, leader); }
+
 // Wave-aggregated update of the states of one group.
 //   pending   : this lane still has a row to merge
-//   tag       : equal tags <=> same target group
+//   tag       : equal tags <=> same target group (32- or 64-bit)
 //   operand(a): 64-bit operand of op a for this lane's row (all lanes)
 //   base(tag) : state pointer of op 0 of group `tag` (leader lane only);
 //               op a lives at base + a * stride
 // Returns true when this lane's row was merged here.  Every lane of the wave
-// must call it with the same nops and min_lanes.
-template <class OPS, class OperandFn, class BaseFn>
-__device__ __forceinline__ bool wave_peel(bool pending, uint64_t tag, const AggSpec& spec, int min_lanes,
+// must call it, with the full wave active and the same min_lanes.  The
+// leader is the lowest pending lane; its tag is read with a scalar lane read.
+template <class OPS, class Tag, class OperandFn, class BaseFn>
+__device__ __forceinline__ bool wave_peel(bool pending, Tag tag, const AggSpec& spec, int min_lanes,
                                           OperandFn operand, BaseFn base, int64_t stride)
 {
     uint64_t pend = __ballot(pending);
@@ -52,12 +80,12 @@ __device__ __forceinline__ bool wave_peel(bool pending, uint64_t tag, const AggS
     for (int it = 0; it < kPeelTries; ++it) {
         if (pend == 0) break;
         const int leader = __ffsll((unsigned long long)pend) - 1;
-        const uint64_t ltag = __shfl((unsigned long long)tag, leader);
-        const uint64_t m = __ballot(pending && tag == ltag) & pend;
+        const Tag ltag = read_leader(tag, leader);
+        const uint64_t m = __ballot(tag == ltag) & pend;
         pend &= ~m;
         if (__popcll(m) < min_lanes) continue;
         const bool mine = (m >> lane) & 1ull;
-        decltype(base(0ull)) p = nullptr;
+        decltype(base(ltag)) p = nullptr;
         if (lane == leader) p = base(ltag);
         for_each_op<OPS>(spec, [&](auto a) {
             const int32_t cls = OPS::cls(spec, a);

@@ -10,7 +10,7 @@
 namespace eg {
 namespace jit {
 
-enum class Kind { HASH_AGG, PART_AGG };
+enum class Kind { HASH_AGG, PART_AGG, PART_AGG_GATHER };
 
 // The kernel specialised for (kind, key bytes, value layout type name, ops of
 // `spec`), or nullptr while it is not available: JIT off, `rows` below the

@@ -162,11 +162,12 @@ struct LdsTable {
     unsigned long long* st;
     KT* keys;
     unsigned long long* side;
-    int T, cr, rlog;
+    int T, cr, rlog, shift;
 
     __device__ LdsTable(unsigned long long* smem, int nops, int cr_log2, int rep_log2)
     {
         cr = 1 << cr_log2;
+        shift = cr_log2 > 0 ? 32 - cr_log2 : 31;  // a 1-slot table is never probed
         rlog = rep_log2;
         T = (cr + 1) << rlog;
         st = smem;
@@ -178,6 +179,14 @@ struct LdsTable {
         const int T = ((1 << cr_log2) + 1) << rep_log2;
         return nops * T + (T * KB + 7) / 8 + (1 << rep_log2);
     }
+    // Home slot of encoded key e: multiplicative (Fibonacci) hashing, the top
+    // cr_log2 bits of e * 2^32/phi -- one multiply; 8-byte keys fold their
+    // halves first.  Only probe lengths depend on it (1 <= cr_log2 <= 31).
+    __device__ __forceinline__ uint32_t home(KT e) const
+    {
+        const uint32_t x = (KB == 4) ? (uint32_t)e : (uint32_t)e ^ (uint32_t)((uint64_t)e >> 32);
+        return (x * 0x9e3779b1u) >> shift;
+    }
     // Finds or inserts encoded key e in replica r; -1 when probe_limit slots
     // are taken by other keys.
     __device__ __forceinline__ int find(KT e, int r, int probe_limit)
@@ -187,8 +196,7 @@ struct LdsTable {
                 __hip_atomic_store(&side[r], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             return (cr << rlog) | r;
         }
-        uint32_t h = (KB == 4) ? dev::fmix32((uint32_t)e) : (uint32_t)(dev::fmix64((uint64_t)e) >> 32);
-        h &= (uint32_t)(cr - 1);
+        uint32_t h = home(e);
         for (int p = 0; p < probe_limit; ++p) {
             const int idx = (int)(h << rlog) | r;
             KT cur = __hip_atomic_load(&keys[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -240,7 +248,7 @@ __device__ __forceinline__ bool lds_row(LdsTable<KB>& tab, bool use_lds, int r_m
         bool hit = false;
         if (valid) {
             e = (KB == 4) ? (KT)((uint32_t)kraw ^ dev::kSign32) : (KT)dev::enc64(dev::widen_key(kraw, KB, key.is_signed));
-            h = ((KB == 4) ? dev::fmix32((uint32_t)e) : (uint32_t)(dev::fmix64((uint64_t)e) >> 32)) & (uint32_t)(tab.cr - 1);
+            h = tab.home(e);
             const int idx = (int)(h << tab.rlog) | (lane & r_mask);
             const KT cur = __hip_atomic_load(&tab.keys[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             hit = e != 0 && cur == e;
@@ -252,7 +260,7 @@ __device__ __forceinline__ bool lds_row(LdsTable<KB>& tab, bool use_lds, int r_m
     }
     bool done = false;
     if (use_lds && r_mask == 0 && peel_min <= 64) {
-        done = dev::wave_peel<OPS>(slot >= 0, (uint64_t)(int64_t)slot, spec, peel_min,
+        done = dev::wave_peel<OPS>(slot >= 0, (uint32_t)slot, spec, peel_min,
                                    [&](int a) { return dev::op_operand<OPS, NV>(spec, a, cv); },
                               [&](uint64_t s) { return &tab.st[(int64_t)s]; }, (int64_t)tab.T);
     }

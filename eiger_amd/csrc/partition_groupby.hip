@@ -71,14 +71,14 @@ PartTune part_tune()
 using agg::kPBMax;
 using agg::Unit;
 
-template <int KB, class VL>
+template <int KB, class VL, bool GATHER>
 __global__ __launch_bounds__(kPBMax) void part_agg_kernel(KeyDesc key, AggSpec spec, const int64_t* perm,
                                                        const Unit* units, GTable gt, OutSpec os,
                                                        unsigned long long* counter, int64_t capacity,
                                                        int cr_log2, int probe_limit, int peel_min)
 {
-    agg::part_agg<KB, VL, dev::RtOps>(key, spec, perm, units, gt, os, counter, capacity, cr_log2, probe_limit,
-                                      peel_min);
+    agg::part_agg<KB, VL, dev::RtOps, GATHER>(key, spec, perm, units, gt, os, counter, capacity, cr_log2,
+                                              probe_limit, peel_min);
 }
 
 template <int KB>
@@ -131,19 +131,20 @@ int ceil_log2(int64_t x)
     return b;
 }
 
-template <int KB>
+template <int KB, bool GATHER>
 void launch_part_agg(const KeyDesc& key, const AggSpec& spec, const int64_t* perm, const Unit* units, int nunits,
                      const GTable& gt, const OutSpec& os, unsigned long long* counter, int64_t capacity, int cr_log2,
                      size_t lds, int block, int64_t rows, hipStream_t s)
 {
     with_vlayout(spec, true, [&](auto vl) {
         using VL = decltype(vl);
-        auto kern = part_agg_kernel<KB, VL>;
+        auto kern = part_agg_kernel<KB, VL, GATHER>;
         if (enable_dynamic_lds((const void*)kern) < lds) raise(EG_ERR_UNSUPPORTED, "partition LDS table too large");
         KernelTimer timer("partition_agg", s);
         int probe_limit = 64, peel_min = 8;
         // the kernel with this request's ops compiled in, once it is available
-        if (hipFunction_t f = jit::lookup(jit::Kind::PART_AGG, KB, jit::vlayout_name<VL>(), spec, rows)) {
+        const jit::Kind kind = GATHER ? jit::Kind::PART_AGG_GATHER : jit::Kind::PART_AGG;
+        if (hipFunction_t f = jit::lookup(kind, KB, jit::vlayout_name<VL>(), spec, rows)) {
             KeyDesc a_key = key;
             AggSpec a_spec = spec;
             const int64_t* a_perm = perm;
@@ -161,6 +162,19 @@ void launch_part_agg(const KeyDesc& key, const AggSpec& spec, const int64_t* per
                            cr_log2, probe_limit, peel_min);
         EG_LAUNCH_CHECK();
     });
+}
+
+template <int KB>
+void launch_part(bool gather, const KeyDesc& key, const AggSpec& spec, const int64_t* perm, const Unit* units,
+                 int nunits, const GTable& gt, const OutSpec& os, unsigned long long* counter, int64_t capacity,
+                 int cr_log2, size_t lds, int block, int64_t rows, hipStream_t s)
+{
+    if (gather)
+        launch_part_agg<KB, true>(key, spec, perm, units, nunits, gt, os, counter, capacity, cr_log2, lds, block,
+                                  rows, s);
+    else
+        launch_part_agg<KB, false>(key, spec, perm, units, nunits, gt, os, counter, capacity, cr_log2, lds, block,
+                                   rows, s);
 }
 
 }  // namespace
@@ -331,11 +345,11 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
         DevBuf counter(8, s);
         fill_zero(counter.get(), 8, s);
         if (kb == 4)
-            launch_part_agg<4>(pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
-                               counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, n, s);
+            launch_part<4>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
+                           counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, n, s);
         else
-            launch_part_agg<8>(pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
-                               counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, n, s);
+            launch_part<8>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
+                           counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, n, s);
         int64_t meta[3];
         ht.read_meta(meta, s);
         const bool table_full = meta[2] != 0;
