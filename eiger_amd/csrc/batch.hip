@@ -119,11 +119,11 @@ __global__ __launch_bounds__(kBatchBlock) void batch_agg_kernel(const BQuery* __
                                                                 AggSpec spec, int key_signed, int vbytes, int W,
                                                                 int cr_log2, int probe_limit, int peel_min)
 {
-    extern __shared__ unsigned long long smem[];
+    extern __shared__ __align__(16) unsigned long long smem[];
     __shared__ int s_item;
     const int nops = spec.nops;
     const int lane = dev::lane_id();
-    LdsTable<KB> tab(smem, nops, cr_log2, 0);
+    LdsTable<KB, true> tab(smem, nops, cr_log2, 0);
     const int words = LdsTable<KB>::words(nops, cr_log2, 0);
     uint32_t ins = 0;
     KeyDesc key;
@@ -401,10 +401,7 @@ BatchPlan* batch_plan_create(const eg_batch_query* queries, int64_t nq, int32_t 
 
     const DeviceInfo& di = device_info();
     int cr = T.lds_slots_log2;
-    auto lds_of = [&](int c) {
-        const size_t TT = (size_t)(1 << c) + 1;
-        return (size_t)p->spec.nops * TT * 8 + (TT * p->kb + 7) / 8 * 8 + 8;
-    };
+    auto lds_of = [&](int c) { return (size_t)lds::lds_table_words(p->spec.nops, c, 0, p->kb) * 8; };
     while (cr > 6 && lds_of(cr) + 256 > (size_t)di.max_lds) --cr;
     p->tuning.lds_slots_log2 = cr;
     p->lds = lds_of(cr);

@@ -86,7 +86,7 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
 {
     constexpr int NV = VL::NV > 0 ? VL::NV : 1;
     using KT = typename LdsKey<KB>::T;
-    extern __shared__ unsigned long long smem[];
+    extern __shared__ __align__(16) unsigned long long smem[];
     const bool use_lds = cr_log2 >= 0;
     const int nops = OPS::n(spec);
     const int lane = dev::lane_id();
@@ -176,11 +176,11 @@ __device__ __forceinline__ void part_agg(const KeyDesc& key, const AggSpec& spec
     constexpr int NV = VL::NV > 0 ? VL::NV : 1;
     constexpr int J = 4;  // rows per lane per step: independent loads in flight
     using KT = typename LdsKey<KB>::T;
-    extern __shared__ unsigned long long smem[];
+    extern __shared__ __align__(16) unsigned long long smem[];
     __shared__ unsigned long long sh[17];
     const int nops = OPS::n(spec);
     const int lane = dev::lane_id();
-    LdsTable<KB> tab(smem, nops, cr_log2, 0);
+    LdsTable<KB, true> tab(smem, nops, cr_log2, 0);  // keys of a unit share hash bits: bucketed probing
     const int words = LdsTable<KB>::words(nops, cr_log2, 0);
     for (int i = threadIdx.x; i < words; i += blockDim.x) smem[i] = 0ull;
     __syncthreads();
@@ -232,30 +232,60 @@ __device__ __forceinline__ void part_agg(const KeyDesc& key, const AggSpec& spec
     }
     __syncthreads();
     // finished groups of this unit
-    for (int base = 0; base <= tab.cr; base += blockDim.x) {
-        const int s = base + threadIdx.x;
-        bool occ = false;
-        KT kw = 0;
-        if (s < tab.cr) {
-            kw = tab.keys[s];
-            occ = kw != 0;
-        } else if (s == tab.cr) {
-            occ = tab.side[0] != 0ull;
+    auto slot_key = [&](int sl, KT& kw) {
+        kw = 0;
+        if (sl < tab.cr) {
+            kw = tab.keys[sl];
+            return kw != 0;
         }
-        const uint64_t enc = lds_to_enc64<KB>(kw, key.is_signed);
-        if (u.sliced) {
-            if (occ) {
-                unsigned long long* g = gt_slot(gt, enc, ins);
-                if (g)
-                    dev::merge_states<OPS>(spec, [&](int a) { return g + 1 + a; },
-                                      [&](int a) { return tab.st[(size_t)a * tab.T + s]; });
+        return sl == tab.cr && tab.side[0] != 0ull;
+    };
+    if (u.sliced) {
+        // the partition continues in other units: merge into the HBM table
+        for (int sl = threadIdx.x; sl <= tab.cr; sl += blockDim.x) {
+            KT kw;
+            if (!slot_key(sl, kw)) continue;
+            unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed), ins);
+            if (g)
+                dev::merge_states<OPS>(spec, [&](int a) { return g + 1 + a; },
+                                       [&](int a) { return tab.st[(size_t)a * tab.T + sl]; });
+        }
+    } else {
+        // Written straight out.  One device atomic per unit claims the
+        // unit's output range (a same-address atomic per chunk of slots
+        // serialises across the ~10^5 units); each wave then owns a
+        // contiguous run of it.
+        const int w = threadIdx.x >> 6;
+        const int nw = (blockDim.x + 63) >> 6;
+        uint32_t wave_groups = 0;
+        for (int base = 0; base <= tab.cr; base += blockDim.x) {
+            KT kw;
+            wave_groups += (uint32_t)__popcll(__ballot(slot_key(base + threadIdx.x, kw)));
+        }
+        if (lane == 0) sh[w] = wave_groups;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long tot = 0;
+            for (int i = 0; i < nw; ++i) {
+                const unsigned long long c = sh[i];
+                sh[i] = tot;
+                tot += c;
             }
-        } else {
-            const int64_t idx = dev::block_claim(counter, occ, sh);
+            sh[16] = tot ? atomicAdd(counter, tot) : 0ull;
+        }
+        __syncthreads();
+        int64_t pos = (int64_t)(sh[16] + sh[w]);
+        for (int base = 0; base <= tab.cr; base += blockDim.x) {
+            const int sl = base + threadIdx.x;
+            KT kw;
+            const bool occ = slot_key(sl, kw);
+            const uint64_t m = __ballot(occ);
             if (occ) {
-                if (idx < capacity) dev::emit_group(os, idx, enc, tab.st + s, (int64_t)tab.T);
+                const int64_t idx = pos + __popcll(m & dev::lanemask_lt());
+                if (idx < capacity) dev::emit_group(os, idx, lds_to_enc64<KB>(kw, key.is_signed), tab.st + sl, (int64_t)tab.T);
                 else atomicExch(&gt.meta[2], 1ull);  // output capacity exceeded: retry larger
             }
+            pos += __popcll(m);
         }
     }
     gt_count_inserts(gt, ins);

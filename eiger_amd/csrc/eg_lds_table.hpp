@@ -152,13 +152,37 @@ __device__ __forceinline__ void load_tile(const KeyDesc& key, int kb, const AggS
 
 // Layout of the dynamic LDS (8-byte words), T = (Cr + 1) * R slots:
 //   states: nops * T words; op a of slot idx at [a * T + idx]
-//   keys:   T entries of KB bytes (encoded; 0 = empty)
+//   keys:   T entries of KB bytes (encoded; 0 = empty), 16-byte aligned
 //   side:   R words, side slot of replica r used
 // Slot (s, r) has index (s << rlog) | r; s = Cr is the side slot holding the
-// key whose LDS encoding is zero.
-template <int KB>
+// key whose LDS encoding is zero.  The dynamic LDS base must be 16-byte
+// aligned.
+__host__ __device__ constexpr int lds_keys_word(int nops, int T) { return (nops * T + 1) & ~1; }
+
+__host__ __device__ constexpr int lds_table_words(int nops, int cr_log2, int rep_log2, int kb)
+{
+    return lds_keys_word(nops, ((1 << cr_log2) + 1) << rep_log2) +
+           ((((1 << cr_log2) + 1) << rep_log2) * kb + 7) / 8 + (1 << rep_log2);
+}
+
+// Probing of the unreplicated table (R = 1) reads keys kBucket at a time
+// from aligned 16-byte groups of slots, so a wave's probe loop -- as long as
+// its worst lane's displacement -- takes a few iterations at load 1/2 instead
+// of tens.  Two probe orders, fixed per table:
+//   bucketed: b, b+1, .., b+3, then the next group (b = h rounded down); a
+//             key is in its home group ~97% of the time at load 1/2, and the
+//             first probe reads that whole group.  For tables of hash-scattered
+//             keys (partition units).
+//   linear:   h, h+1, ...; the first probe reads slot h alone.  Dense key
+//             ranges sit exactly at home under multiplicative hashing, so the
+//             single-slot read always hits.  For the hash kernel.
+constexpr int kBucket = 4;
+
+// BUCKETED: the bucketed probe order (the table is then never replicated).
+template <int KB, bool BUCKETED = false>
 struct LdsTable {
     using KT = typename LdsKey<KB>::T;
+    static constexpr bool bucketed = BUCKETED;
     unsigned long long* st;
     KT* keys;
     unsigned long long* side;
@@ -171,13 +195,12 @@ struct LdsTable {
         rlog = rep_log2;
         T = (cr + 1) << rlog;
         st = smem;
-        keys = reinterpret_cast<KT*>(st + (size_t)nops * T);
-        side = st + (size_t)nops * T + ((size_t)T * KB + 7) / 8;
+        keys = reinterpret_cast<KT*>(st + lds_keys_word(nops, T));
+        side = st + lds_keys_word(nops, T) + ((size_t)T * KB + 7) / 8;
     }
     __device__ static int words(int nops, int cr_log2, int rep_log2)
     {
-        const int T = ((1 << cr_log2) + 1) << rep_log2;
-        return nops * T + (T * KB + 7) / 8 + (1 << rep_log2);
+        return lds_table_words(nops, cr_log2, rep_log2, KB);
     }
     // Home slot of encoded key e: multiplicative (Fibonacci) hashing, the top
     // cr_log2 bits of e * 2^32/phi -- one multiply; 8-byte keys fold their
@@ -187,8 +210,23 @@ struct LdsTable {
         const uint32_t x = (KB == 4) ? (uint32_t)e : (uint32_t)e ^ (uint32_t)((uint64_t)e >> 32);
         return (x * 0x9e3779b1u) >> shift;
     }
+    // The kBucket keys from slot b on (R = 1, b a multiple of kBucket).
+    __device__ __forceinline__ void read_bucket(uint32_t b, KT (&k)[kBucket]) const
+    {
+        if constexpr (KB == 4) {
+            const uint4 v = *reinterpret_cast<const uint4*>(keys + b);
+            k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+        } else {
+            const uint4 v0 = reinterpret_cast<const uint4*>(keys + b)[0];
+            const uint4 v1 = reinterpret_cast<const uint4*>(keys + b)[1];
+            k[0] = v0.x | ((unsigned long long)v0.y << 32); k[1] = v0.z | ((unsigned long long)v0.w << 32);
+            k[2] = v1.x | ((unsigned long long)v1.y << 32); k[3] = v1.z | ((unsigned long long)v1.w << 32);
+        }
+    }
     // Finds or inserts encoded key e in replica r; -1 when probe_limit slots
-    // are taken by other keys.
+    // are taken by other keys.  Every inserter of a key walks the same slot
+    // sequence and keys never leave a slot, so a key lands in exactly one
+    // slot: the first one of its sequence that was empty.
     __device__ __forceinline__ int find(KT e, int r, int probe_limit)
     {
         if (e == 0) {
@@ -197,6 +235,29 @@ struct LdsTable {
             return (cr << rlog) | r;
         }
         uint32_t h = home(e);
+        if (BUCKETED || (rlog == 0 && cr >= kBucket)) {
+            if constexpr (BUCKETED) h &= ~(uint32_t)(kBucket - 1);
+            for (int left = probe_limit; left > 0;) {
+                const uint32_t b = h & ~(uint32_t)(kBucket - 1);
+                const int i0 = (int)(h - b);
+                KT k[kBucket];
+                read_bucket(b, k);
+                int at = -1;
+#pragma unroll
+                for (int i = kBucket - 1; i >= 0; --i)
+                    if ((BUCKETED || i >= i0) && k[i] == e) at = i;
+                if (at >= 0) return (int)b + at;
+#pragma unroll
+                for (int i = 0; i < kBucket; ++i) {
+                    if ((!BUCKETED && i < i0) || k[i] != 0) continue;
+                    const KT prev = atomicCAS(&keys[b + i], (KT)0, e);
+                    if (prev == 0 || prev == e) return (int)b + i;
+                }
+                left -= kBucket - i0;
+                h = (b + kBucket) & (uint32_t)(cr - 1);
+            }
+            return -1;
+        }
         for (int p = 0; p < probe_limit; ++p) {
             const int idx = (int)(h << rlog) | r;
             KT cur = __hip_atomic_load(&keys[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -234,8 +295,8 @@ __device__ __forceinline__ void canon_row(const AggSpec& spec, const uint64_t (&
 }
 
 // One row into the LDS table; returns true when it needs the HBM table.
-template <class OPS, int KB, int NV>
-__device__ __forceinline__ bool lds_row(LdsTable<KB>& tab, bool use_lds, int r_mask, int lane, const KeyDesc& key,
+template <class OPS, int KB, int NV, bool BK>
+__device__ __forceinline__ bool lds_row(LdsTable<KB, BK>& tab, bool use_lds, int r_mask, int lane, const KeyDesc& key,
                                         const AggSpec& spec, uint64_t kraw, const uint64_t (&cv)[NV],
                                         bool valid, int probe_limit, int peel_min)
 {
@@ -249,10 +310,23 @@ __device__ __forceinline__ bool lds_row(LdsTable<KB>& tab, bool use_lds, int r_m
         if (valid) {
             e = (KB == 4) ? (KT)((uint32_t)kraw ^ dev::kSign32) : (KT)dev::enc64(dev::widen_key(kraw, KB, key.is_signed));
             h = tab.home(e);
-            const int idx = (int)(h << tab.rlog) | (lane & r_mask);
-            const KT cur = __hip_atomic_load(&tab.keys[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            hit = e != 0 && cur == e;
-            if (hit) slot = idx;
+            if constexpr (BK) {
+                // the whole home group in one read
+                const uint32_t b = h & ~(uint32_t)(kBucket - 1);
+                KT k[kBucket];
+                tab.read_bucket(b, k);
+                int at = -1;
+#pragma unroll
+                for (int i = kBucket - 1; i >= 0; --i)
+                    if (k[i] == e) at = i;
+                hit = e != 0 && at >= 0;
+                if (hit) slot = (int)b + at;
+            } else {
+                const int idx = (int)(h << tab.rlog) | (lane & r_mask);
+                const KT cur = __hip_atomic_load(&tab.keys[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                hit = e != 0 && cur == e;
+                if (hit) slot = idx;
+            }
         }
         if (__ballot(valid && !hit) != 0ull) {
             if (valid && !hit) slot = tab.find(e, lane & r_mask, probe_limit);

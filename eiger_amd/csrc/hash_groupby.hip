@@ -124,8 +124,7 @@ HashLayout plan_hash_layout(int nops, int key_bytes, int64_t group_bound, const 
     const DeviceInfo& di = device_info();
     HashLayout L;
     auto bytes_of = [&](int cr_log2, int rep_log2) {
-        const size_t T = (size_t)((1 << cr_log2) + 1) << rep_log2;
-        return (size_t)nops * T * 8 + (T * key_bytes + 7) / 8 * 8 + ((size_t)1 << rep_log2) * 8;
+        return (size_t)lds::lds_table_words(nops, cr_log2, rep_log2, key_bytes) * 8;
     };
     const size_t budget = (size_t)di.max_lds;
     // replicas: the most (<= 64) that still give every replica >= 2x the group bound

@@ -205,10 +205,7 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
         }
     }
     int cr_log2 = tune.cr_log2;
-    auto lds_of = [&](int c) {
-        const size_t T = (size_t)(1 << c) + 1;
-        return (size_t)nops * T * 8 + (T * kb + 7) / 8 * 8 + 8;
-    };
+    auto lds_of = [&](int c) { return (size_t)lds_table_words(nops, c, 0, kb) * 8; };
     while (cr_log2 > 6 && lds_of(cr_log2) + 256 > (size_t)di.max_lds) --cr_log2;  // 256: static LDS
     const size_t lds = lds_of(cr_log2);
     const int64_t per_part = (int64_t)1 << (cr_log2 - 1);  // target groups per partition (load 1/2)

@@ -196,6 +196,21 @@ def test_groups_beyond_the_lds_table_spill_to_hbm(name, fn):
     assert_same(got, O.groupby(keys, [vals], reqs))
 
 
+@pytest.mark.parametrize("name,fn", list(STRATEGIES.items()))
+def test_keys_sharing_one_lds_home_slot(name, fn):
+    """8-byte keys (a << 32) | a fold to 0 in the LDS table's hash, so every
+    key has the same home slot: probe runs hit their limit and rows spill to
+    the HBM table.  Results stay exact."""
+    n = 1 << 18
+    rng = np.random.default_rng(17)
+    a = rng.choice(np.arange(1, 1 << 31, dtype=np.int64), size=3000, replace=False)
+    keys = ((a << 32) | a)[rng.integers(0, 3000, size=n)]
+    vals = O.gen_values(n, O.VAL_UNIFORM_I64, np.int64, seed=17, stream_id=2)
+    reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MAX)]
+    got = sorted_by_key(run(fn, keys, [vals], reqs, G.groupby_hint(groups_upper_bound=3000)))
+    assert_same(got, O.groupby(keys, [vals], reqs))
+
+
 @pytest.mark.parametrize("name", ["hash", "partition", "partition_gfur"])
 def test_group_bound_far_too_small_retries(name):
     """A groups_upper_bound far below the truth fills the HBM table / output;
