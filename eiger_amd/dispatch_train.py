@@ -5,10 +5,12 @@
 1. Queries are drawn like the two benchmark workloads: (a) batched uneven
    queries -- rows log-uniform in [1e3, 4e6], distinct count log-uniform in
    [1, rows/2], uniform or Zipf s ~ U[0.5, 1.5], SUM(int64) + COUNT; (b)
-   strategy-sweep queries -- rows log-uniform in [2^20, 2^23], distinct count
-   log-uniform in [2^4, 2^26] (capped by rows), uniform or Zipf s in {0.5, 1,
-   1.5}, COUNT + SUM/MIN/MAX(int64) + SUM(float32).  Keys are int32, generated
-   on the device.
+   strategy-sweep queries -- rows log-uniform in [2^20, 2^23] (one in ten in
+   [2^24, 2^28]), distinct count log-uniform in [2^4, 2^26] (capped by rows),
+   uniform or Zipf s in {0.5, 1, 1.5}, COUNT + SUM/MIN/MAX(int64) +
+   SUM(float32).  Keys are int32, generated on the device.  The aggregation
+   kernels specialised for each request list are compiled before timing, as
+   a long-running caller would have them.
 2. Each query's 16 sample features come from the HIP feature kernel; each
    strategy (hash, sort, partition) is timed on it, with the true group count
    as the hint (the label is what each strategy costs when sized right).
@@ -60,7 +62,7 @@ def draw_queries(n: int, seed: int):
             s = 0.0 if rng.random() < 0.5 else float(rng.uniform(0.5, 1.5))
             out.append(("batch", rows, max(d, 1), s))
         else:
-            rows = int(2 ** rng.uniform(20, 23))
+            rows = int(2 ** (rng.uniform(24, 28) if rng.random() < 0.1 else rng.uniform(20, 23)))
             d = int(2 ** rng.uniform(4, 26))
             d = max(1, min(d, rows))
             s = float(rng.choice([0.0, 0.5, 1.0, 1.5]))
@@ -208,8 +210,11 @@ def main():
     specs_tr = draw_queries(args.train, seed=1)
     specs_ho = draw_queries(args.heldout, seed=2)
     wl = Workload(max(s[1] for s in specs_tr + specs_ho))
-    # warm every strategy once
-    for spec in specs_tr[:8]:
+    G.jit_configure(G.jit_mode.SYNC, None)  # specialised kernels compile on first use, before timing
+    # warm every strategy once, on both request lists at sizes that use the
+    # specialised kernels
+    warm = [("batch", 1 << 21, 1000, 0.0), ("sweep", 1 << 21, 1000, 0.0), ("sweep", 1 << 21, 1 << 20, 0.0)]
+    for spec in warm + specs_tr[:8]:
         k, vals, reqs, _, _ = wl.query(spec, 0)
         for name in STRATS:
             FNS[name]([k], vals, reqs, G.groupby_hint(groups_upper_bound=spec[2])).free()

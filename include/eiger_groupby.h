@@ -366,7 +366,7 @@ enum { EG_JIT_OFF = 0, EG_JIT_ASYNC = 1, EG_JIT_SYNC = 2 };
 /* mode: EG_JIT_ASYNC (default; compile on a background thread, use the
  * generic kernel until the specialised one is ready), EG_JIT_SYNC (compile on
  * first use), EG_JIT_OFF, or -1 to keep the mode.  min_rows: inputs with fewer
- * rows use the generic kernel (default 2^22; -1 keeps the value).  The
+ * rows use the generic kernel (default 2^20; -1 keeps the value).  The
  * environment variables EG_JIT (off|sync|async) and EG_JIT_MIN_ROWS set the
  * initial values. */
 int eg_jit_configure(int32_t mode, int64_t min_rows);

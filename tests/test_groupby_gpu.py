@@ -366,6 +366,36 @@ def test_full_size_sweep_shape_invariants():
     assert abs(float(fs.astype(np.float64).sum()) - float(tot[1][0])) / float(tot[1][0]) < 1e-6
 
 
+def test_partition_heavy_full_size_properties():
+    """2^30 rows of int64 keys, each of 2^28 keys exactly 4 times (uniform
+    exact count), SUM(int64) + COUNT with the default hint (group count
+    unknown): 2^28 groups whose keys are exactly [0, 2^28) with no duplicate,
+    every count 4, and the sums adding up to the value column's total.
+    Checked on the device with the scalar aggregate and a second group-by."""
+    n, d = 1 << 30, 1 << 28
+    k = G.gen_keys_uniform(n, d, seed=42, t=G.type_id.INT64)
+    v = G.gen_values(n, G.val_dist.UNIFORM_I64, G.type_id.INT64, seed=42, stream_id=2)
+    t = G.groupby([k], [v], [(0, A.SUM), (-1, A.COUNT_ALL)], G.groupby_hint())
+    k.free()
+    assert t.num_rows() == d
+    kk, ss, cc = t.get_column(0), t.get_column(1), t.get_column(2)
+
+    def scalar(col, reqs):
+        r = G.aggregate([col], reqs)
+        out = [int(c[0]) for c in r.to_numpy()]
+        r.free()
+        return out
+
+    assert scalar(kk, [(0, A.MIN), (0, A.MAX)]) == [0, d - 1]
+    assert scalar(cc, [(0, A.MIN), (0, A.MAX), (0, A.SUM)]) == [4, 4, n]
+    assert scalar(ss, [(0, A.SUM)]) == scalar(v, [(0, A.SUM)])
+    v.free()
+    again = G.groupby([kk], [], [(-1, A.COUNT_ALL)], G.groupby_hint(groups_upper_bound=d))
+    assert again.num_rows() == d  # no key appears twice
+    again.free()
+    t.free()
+
+
 @pytest.mark.parametrize("name", ["auto", "hash", "partition"])
 def test_wide_request_lists_are_split_into_passes(name):
     """More aggregation states (or value columns) than one pass holds: the
