@@ -341,6 +341,21 @@ __device__ __forceinline__ void emit_group(const OutSpec& os, int64_t g, uint64_
     }
 }
 
+// ---- XCD-aware block order ------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b mod 8), and each XCD has its own L2.  xcd_remap(b, n) is a bijection on
+// [0, n) giving XCD x the contiguous range of work items
+// [x*n/8, (x+1)*n/8) (rounded), in order, so neighbouring items -- whose
+// output cache lines overlap -- run on one XCD and their partial lines merge
+// in that L2 instead of leaving as partial writes from two XCDs.
+constexpr uint32_t kXcds = 8;
+
+__host__ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n)
+{
+    const uint32_t x = b % kXcds, k = b / kXcds, q = n / kXcds, r = n % kXcds;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // ---- wave helpers ----------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 

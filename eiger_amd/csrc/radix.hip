@@ -3,9 +3,10 @@
 //
 // The reference delegates these to cub::DeviceRadixSort / DeviceHistogram /
 // DeviceScan (operators/sort/sort.cu:15-66, operators/partition/partition.cu:
-// 24-171).  Here one pass over 8 digit bits is:
+// 24-171).  Here one pass over up to 8 digit bits is:
 //   radix_hist_kernel    per 4096-row tile, an LDS histogram of the digit;
-//   scan                 exclusive prefix over [digit][tile] (device-wide);
+//   scan                 exclusive prefix over [digit][tile] (device-wide,
+//                        the 2^bits digits of the pass only);
 //   radix_scatter_kernel per tile, a stable rank of every row among the
 //                        tile's rows of the same digit (each wave ranks its own
 //                        1024 consecutive rows with wave64 ballots on the digit
@@ -92,7 +93,9 @@ __global__ __launch_bounds__(kRB) void radix_hist_kernel(RadixKeys k, int64_t n,
         if (e < n) atomicAdd(&h[digit_of(k, raw[i], shift, mask)], 1u);
     }
     __syncthreads();
-    hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
+    // [digit][tile], only the 2^bits digits in use: each is a lone 4-byte
+    // store into its own digit row
+    if ((uint32_t)t <= mask) hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
 }
 
 // exclusive scan of 256 values held one per thread (256 threads)
@@ -113,27 +116,45 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wsu
     return x - v + add;
 }
 
-// Writes one column of the tile in digit order: values go to their tile
-// position in LDS, then leave in runs of one digit (coalesced stores).
-// `vals` holds the row's value when `have` (the keys, already in registers).
+// A column of the tile leaves in digit order in three steps: its values are
+// loaded into registers (load_column), put at their tile position in LDS
+// (stage_column), and written out in runs of one digit (store_column,
+// coalesced stores).  The next column's loads are issued before the current
+// column's stores, so their latency overlaps.
 template <int B>
-__device__ __forceinline__ void scatter_column(const void* src, void* dst, bool rid, bool have,
-                                               const uint64_t (&vals)[kRI], const uint32_t (&pos)[kRI],
-                                               int64_t wbase, int lane, int t, int32_t tcount,
-                                               unsigned long long* stage, const uint8_t* sdig,
-                                               const uint32_t* gofs, const uint32_t* lstart)
+__device__ __forceinline__ void load_column_w(const void* src, bool rid, const uint32_t (&pos)[kRI], int64_t wbase,
+                                              int lane, uint64_t (&v)[kRI])
 {
-    __syncthreads();
-    uint64_t v[kRI];
 #pragma unroll
     for (int i = 0; i < kRI; ++i) {
         const int64_t e = wbase + (int64_t)i * 64 + lane;
-        v[i] = have ? vals[i] : (rid ? (uint64_t)e : (pos[i] != 0xffffffffu ? load_key<B>(src, e) : 0ull));
+        v[i] = rid ? (uint64_t)e : (pos[i] != 0xffffffffu ? load_key<B>(src, e) : 0ull);
     }
+}
+
+__device__ __forceinline__ void load_column(const void* src, int bytes, bool rid, const uint32_t (&pos)[kRI],
+                                            int64_t wbase, int lane, uint64_t (&v)[kRI])
+{
+    switch (rid ? 8 : bytes) {  // uniform; the loads of each case are independent
+    case 1: load_column_w<1>(src, false, pos, wbase, lane, v); break;
+    case 2: load_column_w<2>(src, false, pos, wbase, lane, v); break;
+    case 4: load_column_w<4>(src, false, pos, wbase, lane, v); break;
+    default: load_column_w<8>(src, rid, pos, wbase, lane, v); break;
+    }
+}
+
+__device__ __forceinline__ void stage_column(const uint64_t (&v)[kRI], const uint32_t (&pos)[kRI],
+                                             unsigned long long* stage)
+{
 #pragma unroll
     for (int i = 0; i < kRI; ++i)
         if (pos[i] != 0xffffffffu) stage[pos[i]] = v[i];
-    __syncthreads();
+}
+
+template <int B>
+__device__ __forceinline__ void store_column_w(void* dst, int t, int32_t tcount, const unsigned long long* stage,
+                                               const uint8_t* sdig, const uint32_t* gofs, const uint32_t* lstart)
+{
 #pragma unroll
     for (int i = 0; i < kRI; ++i) {
         const int p = i * kRB + t;
@@ -141,6 +162,18 @@ __device__ __forceinline__ void scatter_column(const void* src, void* dst, bool 
             const uint32_t d = sdig[p];
             store_key<B>(dst, (int64_t)gofs[d] + (p - (int64_t)lstart[d]), stage[p]);
         }
+    }
+}
+
+__device__ __forceinline__ void store_column(void* dst, int bytes, int t, int32_t tcount,
+                                             const unsigned long long* stage, const uint8_t* sdig,
+                                             const uint32_t* gofs, const uint32_t* lstart)
+{
+    switch (bytes) {
+    case 1: store_column_w<1>(dst, t, tcount, stage, sdig, gofs, lstart); break;
+    case 2: store_column_w<2>(dst, t, tcount, stage, sdig, gofs, lstart); break;
+    case 4: store_column_w<4>(dst, t, tcount, stage, sdig, gofs, lstart); break;
+    default: store_column_w<8>(dst, t, tcount, stage, sdig, gofs, lstart); break;
     }
 }
 
@@ -167,10 +200,13 @@ __global__ __launch_bounds__(kRB) void radix_scatter_kernel(RadixKeys k, void* _
 
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const uint32_t mask = (1u << bits) - 1u;
-    const int64_t tbase = (int64_t)blockIdx.x * kRT;
+    // tile order per XCD: the runs of one digit from neighbouring tiles are
+    // adjacent in the output and share their boundary cache lines
+    const uint32_t tile = dev::xcd_remap(blockIdx.x, (uint32_t)ntiles);
+    const int64_t tbase = (int64_t)tile * kRT;
 #pragma unroll
     for (int w = 0; w < kW; ++w) whist[w][t] = 0u;
-    gofs[t] = offs[(int64_t)t * ntiles + blockIdx.x];
+    gofs[t] = t < (1 << bits) ? offs[(int64_t)t * ntiles + tile] : 0u;
 
     const int64_t wbase = tbase + (int64_t)wave * kWR;
     uint64_t kr[kRI];
@@ -229,24 +265,19 @@ __global__ __launch_bounds__(kRB) void radix_scatter_kernel(RadixKeys k, void* _
         }
     }
 
-    // keys (from registers), then every payload column, through the LDS stage
-    scatter_column<KB>(nullptr, kout, false, true, kr, pos, wbase, lane, t, tcount, stage, sdig, gofs, lstart);
+    // keys (from registers), then every payload column, through the LDS stage;
+    // column c+1 is being loaded while column c is stored
+    uint64_t v[kRI];
+    stage_column(kr, pos, stage);
+    __syncthreads();
+    if (pay.n > 0) load_column(pay.in[0], pay.bytes[0], pay.rowid[0] != 0, pos, wbase, lane, v);
+    store_column(kout, KB, t, tcount, stage, sdig, gofs, lstart);
     for (int c = 0; c < pay.n; ++c) {
-        const bool rid = pay.rowid[c] != 0;
-        switch (rid ? 8 : pay.bytes[c]) {
-        case 1:
-            scatter_column<1>(pay.in[c], pay.out[c], false, false, kr, pos, wbase, lane, t, tcount, stage, sdig, gofs, lstart);
-            break;
-        case 2:
-            scatter_column<2>(pay.in[c], pay.out[c], false, false, kr, pos, wbase, lane, t, tcount, stage, sdig, gofs, lstart);
-            break;
-        case 4:
-            scatter_column<4>(pay.in[c], pay.out[c], false, false, kr, pos, wbase, lane, t, tcount, stage, sdig, gofs, lstart);
-            break;
-        default:
-            scatter_column<8>(pay.in[c], pay.out[c], rid, false, kr, pos, wbase, lane, t, tcount, stage, sdig, gofs, lstart);
-            break;
-        }
+        __syncthreads();  // the stage is free again
+        stage_column(v, pos, stage);
+        __syncthreads();
+        if (c + 1 < pay.n) load_column(pay.in[c + 1], pay.bytes[c + 1], pay.rowid[c + 1] != 0, pos, wbase, lane, v);
+        store_column(pay.out[c], pay.rowid[c] != 0 ? 8 : pay.bytes[c], t, tcount, stage, sdig, gofs, lstart);
     }
 }
 
@@ -391,7 +422,8 @@ void radix_pass(const RadixKeys& k, void* keys_out, const std::vector<Payload>& 
     if ((int)pay.size() > kMaxPay) raise(EG_ERR_UNSUPPORTED, "radix pass: too many payload columns");
     const int ntiles = (int)((n + kRT - 1) / kRT);
     const uint32_t mask = (1u << bits) - 1u;
-    DevBuf hist((size_t)256 * ntiles * 4, s), offs((size_t)256 * ntiles * 4, s);
+    const int64_t nh = ((int64_t)1 << bits) * ntiles;  // [digit][tile] counts
+    DevBuf hist((size_t)nh * 4, s), offs((size_t)nh * 4, s);
     {
         KernelTimer timer("radix_hist", s);
         auto hk = k.bytes == 1 ? radix_hist_kernel<1> : k.bytes == 2 ? radix_hist_kernel<2>
@@ -401,7 +433,7 @@ void radix_pass(const RadixKeys& k, void* keys_out, const std::vector<Payload>& 
     }
     {
         KernelTimer timer("radix_scan", s);
-        scan_exclusive_u32(hist.get<uint32_t>(), offs.get<uint32_t>(), (int64_t)256 * ntiles, s, false);
+        scan_exclusive_u32(hist.get<uint32_t>(), offs.get<uint32_t>(), nh, s, false);
     }
     PayArgs pa{};
     pa.n = (int32_t)pay.size();
