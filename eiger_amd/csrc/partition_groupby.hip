@@ -47,6 +47,18 @@ using namespace lds;
 // 2^28 rows takes about half the time of an 8-bit one).
 constexpr int kMaxDigitBits = 6;
 
+// Widest digit of a partitioning pass; EG_PARTITION_DIGIT_BITS (4..8)
+// overrides it for tuning.
+int max_digit_bits()
+{
+    static const int v = [] {
+        const char* e = std::getenv("EG_PARTITION_DIGIT_BITS");
+        const int b = e ? std::atoi(e) : 0;
+        return (b >= 4 && b <= 8) ? b : kMaxDigitBits;
+    }();
+    return v;
+}
+
 // Workgroup size and LDS table slots (log2) of the unit kernel; overridable
 // for tuning with EG_PARTITION_TUNE="slots_log2,threads".
 struct PartTune {
@@ -190,18 +202,27 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     // LDS table and workgroup: 1024 slots with 512 threads runs the unit
     // kernel about 1.5x faster than 2048 slots with 1024 threads (three
     // resident workgroups per CU instead of one), but doubles the partition
-    // count; take it when that costs no extra radix pass.
+    // count; take it when that costs no extra radix pass, widening the digits
+    // from 6 to 7 bits if that is what it takes (a 7-bit pass costs ~7% more
+    // than a 6-bit one; 2^28 rows, D = 2^22: 6.4 ms against 6.7 ms).
     PartTune tune = part_tune();
     auto bits_for = [&](int c) {
         const int64_t pp = (int64_t)1 << (c - 1);
         return std::min(24, std::max(1, ceil_log2((bound + pp - 1) / pp)));
     };
+    int digit_bits = max_digit_bits();
     if (!tune.fixed) {
-        const int p11 = (bits_for(11) + kMaxDigitBits - 1) / kMaxDigitBits;
-        const int p10 = (bits_for(10) + kMaxDigitBits - 1) / kMaxDigitBits;
-        if (p10 == p11 && bound > ((int64_t)1 << 12)) {
-            tune.cr_log2 = 10;
-            tune.block = 512;
+        auto passes = [&](int c, int mb) { return (bits_for(c) + mb - 1) / mb; };
+        const int p11 = passes(11, digit_bits);
+        if (bound > ((int64_t)1 << 12)) {
+            if (passes(10, digit_bits) == p11) {
+                tune.cr_log2 = 10;
+                tune.block = 512;
+            } else if (digit_bits < 7 && passes(10, 7) == p11) {
+                tune.cr_log2 = 10;
+                tune.block = 512;
+                digit_bits = 7;
+            }
         }
     }
     int cr_log2 = tune.cr_log2;
@@ -242,9 +263,9 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
             b.pay_b.push_back(bufs[bufs.size() - 1].get());
         }
     }
-    // balanced digits: ceil(bits / kMaxDigitBits) passes of near-equal width
+    // balanced digits: ceil(bits / digit_bits) passes of near-equal width
     phase_prep.reset();
-    const int passes = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+    const int passes = (bits + digit_bits - 1) / digit_bits;
     const void* pkeys = nullptr;
     std::vector<const void*> ppay;
     {
