@@ -343,8 +343,14 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
                 if (mlp_select(plan.key.data, plan.key_type, plan.rows, h.groups_upper_bound, plan.spec.nops, vbytes, s,
                                &a, &card)) {
                     h.groupby_algo = a;
-                    if (!(h.groups_upper_bound > 0 && h.groups_upper_bound < plan.rows))
-                        learned_bound = std::min<int64_t>(plan.rows, card + card + card / 2 + 1024);
+                    // The model's group-count estimate, with headroom, sizes the
+                    // tables; a caller's bound is an upper bound, often loose
+                    // (skewed keys use few of the possible values), so the
+                    // smaller of the two is taken.  An estimate that is too low
+                    // costs a retry or LDS spills, never a wrong result.
+                    learned_bound = std::min<int64_t>(plan.rows, card + card + card / 2 + 1024);
+                    if (h.groups_upper_bound > 0 && h.groups_upper_bound < learned_bound)
+                        learned_bound = h.groups_upper_bound;
                 } else {
                     int rc = eg_auto_select_algo(keys, num_keys, &h, stream);
                     if (rc != EG_OK) raise(rc, last_error_cstr());

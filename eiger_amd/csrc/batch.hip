@@ -126,6 +126,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_agg_kernel(const BQuery* __
     LdsTable<KB, true> tab(smem, nops, cr_log2, 0);
     const int words = LdsTable<KB>::words(nops, cr_log2, 0);
     uint32_t ins = 0;
+    dev::HotTags hot;  // groups the wave combined last
     KeyDesc key;
     key.bytes = KB;
     key.is_signed = key_signed;
@@ -153,8 +154,8 @@ __global__ __launch_bounds__(kBatchBlock) void batch_agg_kernel(const BQuery* __
                 cv[c] = (valid && spec.nvcols > 0) ? dev::canon(dev::load_raw(q.vals, vbytes, r), spec.cols[0].conv)
                                                    : 0ull;
             bool sp = valid;
-            if (use_lds) sp = lds_row<dev::RtOps, KB, NV>(tab, true, 0, lane, key, spec, kraw, cv, valid, probe_limit, peel_min);
-            if (__ballot(sp) != 0ull) hbm_row<dev::RtOps, KB, NV>(gt, key, spec, sp, kraw, cv, peel_min, ins);
+            if (use_lds) sp = lds_row<dev::RtOps, KB, NV>(tab, true, 0, lane, key, spec, kraw, cv, valid, probe_limit, peel_min, hot);
+            if (__ballot(sp) != 0ull) hbm_row<dev::RtOps, KB, NV>(gt, key, spec, sp, kraw, cv, use_lds ? 65 : peel_min, ins, hot);
         }
         if (use_lds) {
             __syncthreads();

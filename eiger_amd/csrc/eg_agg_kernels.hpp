@@ -88,6 +88,10 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
     using KT = typename LdsKey<KB>::T;
     extern __shared__ __align__(16) unsigned long long smem[];
     const bool use_lds = cr_log2 >= 0;
+    // Rows reach the HBM table as spills of cold keys (hot keys take LDS
+    // slots first), where combining seldom finds partners; only without an
+    // LDS stage do hot keys meet there.
+    const int hbm_peel = use_lds ? 65 : peel_min;
     const int nops = OPS::n(spec);
     const int lane = dev::lane_id();
     LdsTable<KB> tab(smem, nops, use_lds ? cr_log2 : 0, use_lds ? rep_log2 : 0);
@@ -98,6 +102,7 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
     }
     const int r_mask = (1 << (use_lds ? rep_log2 : 0)) - 1;
     uint32_t ins = 0;  // HBM-table insertions by this lane
+    dev::HotTags hot;  // groups the wave combined last
 
     // ---- full tiles: vector loads, next tile in flight while this one runs ----
     const int64_t tile = (int64_t)blockDim.x * kRows;
@@ -116,7 +121,7 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
 #pragma unroll
             for (int c = 0; c < NV; ++c) raw[c] = vcur[c][i];
             canon_row<VL, NV>(spec, raw, cv);
-            if (lds_row<OPS, KB, NV>(tab, use_lds, r_mask, lane, key, spec, kcur[i], cv, true, probe_limit, peel_min))
+            if (lds_row<OPS, KB, NV>(tab, use_lds, r_mask, lane, key, spec, kcur[i], cv, true, probe_limit, peel_min, hot))
                 spill |= 1u << i;
         }
         if (__ballot(spill != 0) != 0ull) {
@@ -128,7 +133,7 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
 #pragma unroll
                 for (int c = 0; c < NV; ++c) raw[c] = pick4(vcur[c], i);
                 canon_row<VL, NV>(spec, raw, cv);
-                hbm_row<OPS, KB, NV>(gt, key, spec, pend, pick4(kcur, i), cv, peel_min, ins);
+                hbm_row<OPS, KB, NV>(gt, key, spec, pend, pick4(kcur, i), cv, hbm_peel, ins, hot);
             }
         }
 #pragma unroll
@@ -148,8 +153,8 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
         for (int c = 0; c < NV; ++c)
             raw[c] = (valid && c < spec.nvcols) ? dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, r) : 0ull;
         canon_row<VL, NV>(spec, raw, cv);
-        const bool sp = lds_row<OPS, KB, NV>(tab, use_lds, r_mask, lane, key, spec, kraw, cv, valid, probe_limit, peel_min);
-        if (__ballot(sp) != 0ull) hbm_row<OPS, KB, NV>(gt, key, spec, sp, kraw, cv, peel_min, ins);
+        const bool sp = lds_row<OPS, KB, NV>(tab, use_lds, r_mask, lane, key, spec, kraw, cv, valid, probe_limit, peel_min, hot);
+        if (__ballot(sp) != 0ull) hbm_row<OPS, KB, NV>(gt, key, spec, sp, kraw, cv, hbm_peel, ins, hot);
     }
     if (use_lds) flush_lds<OPS>(tab, gt, key, spec, rep_log2, r_mask, ins);
     gt_count_inserts(gt, ins);
@@ -187,6 +192,7 @@ __device__ __forceinline__ void part_agg(const KeyDesc& key, const AggSpec& spec
 
     const Unit u = units[blockIdx.x];  // never empty
     uint32_t ins = 0;  // HBM-table insertions by this lane
+    dev::HotTags hot;  // groups the wave combined last
     // Loads of one step (J rows per lane): all issued before any is used;
     // rows past the unit's end read its first row instead (in bounds, no
     // branch) and are masked by `valid`.
@@ -219,8 +225,8 @@ __device__ __forceinline__ void part_agg(const KeyDesc& key, const AggSpec& spec
         for (int j = 0; j < J; ++j) {
             uint64_t cv[NV];
             canon_row<VL, NV>(spec, raw[j], cv);
-            const bool sp = lds_row<OPS, KB, NV>(tab, true, 0, lane, key, spec, kr[j], cv, valid[j], probe_limit, peel_min);
-            if (__ballot(sp) != 0ull) hbm_row<OPS, KB, NV>(gt, key, spec, sp, kr[j], cv, peel_min, ins);
+            const bool sp = lds_row<OPS, KB, NV>(tab, true, 0, lane, key, spec, kr[j], cv, valid[j], probe_limit, peel_min, hot);
+            if (__ballot(sp) != 0ull) hbm_row<OPS, KB, NV>(gt, key, spec, sp, kr[j], cv, 65, ins, hot);  // spills: cold keys
         }
 #pragma unroll
         for (int j = 0; j < J; ++j) {

@@ -63,27 +63,25 @@ __device__ __forceinline__ uint64_t read_leader(uint64_t tag, int leader) { retu
 // Wave-aggregated update of the states of one group.
 //   pending   : this lane still has a row to merge
 //   tag       : equal tags <=> same target group (32- or 64-bit)
+//   hot       : wave-uniform tag of the group last combined (in/out); tried
+//               first, since a hot group stays hot -- a hint only, any value
+//               is correct
 //   operand(a): 64-bit operand of op a for this lane's row (all lanes)
 //   base(tag) : state pointer of op 0 of group `tag` (leader lane only);
 //               op a lives at base + a * stride
 // Returns true when this lane's row was merged here.  Every lane of the wave
-// must call it, with the full wave active and the same min_lanes.  The
-// leader is the lowest pending lane; its tag is read with a scalar lane read.
+// must call it, with the full wave active and the same min_lanes.  Otherwise
+// the leader is the lowest pending lane; its tag is read with a scalar lane
+// read.
 template <class OPS, class Tag, class OperandFn, class BaseFn>
-__device__ __forceinline__ bool wave_peel(bool pending, Tag tag, const AggSpec& spec, int min_lanes,
+__device__ __forceinline__ bool wave_peel(bool pending, Tag tag, Tag& hot, const AggSpec& spec, int min_lanes,
                                           OperandFn operand, BaseFn base, int64_t stride)
 {
     uint64_t pend = __ballot(pending);
     bool done = false;
     const int lane = lane_id();
-#pragma unroll
-    for (int it = 0; it < kPeelTries; ++it) {
-        if (pend == 0) break;
-        const int leader = __ffsll((unsigned long long)pend) - 1;
-        const Tag ltag = read_leader(tag, leader);
-        const uint64_t m = __ballot(tag == ltag) & pend;
-        pend &= ~m;
-        if (__popcll(m) < min_lanes) continue;
+    // merges the rows of lanes m (all with tag ltag) through lane `leader`
+    auto combine_group = [&](uint64_t m, int leader, Tag ltag) {
         const bool mine = (m >> lane) & 1ull;
         decltype(base(ltag)) p = nullptr;
         if (lane == leader) p = base(ltag);
@@ -94,9 +92,34 @@ __device__ __forceinline__ bool wave_peel(bool pending, Tag tag, const AggSpec& 
             if (lane == leader && p) apply_atomic(cls, p + (int)a * stride, v);
         });
         done = done || mine;
+    };
+    if (pend == 0) return false;
+    {
+        const uint64_t m = __ballot(tag == hot) & pend;
+        if (__popcll(m) >= min_lanes) {
+            pend &= ~m;
+            combine_group(m, __ffsll((unsigned long long)m) - 1, hot);
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < kPeelTries; ++it) {
+        if (pend == 0) break;
+        const int leader = __ffsll((unsigned long long)pend) - 1;
+        const Tag ltag = read_leader(tag, leader);
+        const uint64_t m = __ballot(tag == ltag) & pend;
+        pend &= ~m;
+        if (__popcll(m) < min_lanes) continue;
+        combine_group(m, leader, ltag);
+        hot = ltag;
     }
     return done;
 }
+
+// Wave-uniform hints of the groups last combined (see wave_peel).
+struct HotTags {
+    uint32_t lds = 0xffffffffu;
+    uint64_t hbm = ~0ull;
+};
 
 }  // namespace dev
 }  // namespace eg

@@ -204,11 +204,19 @@ struct Result {
 void alloc_outputs(const Plan& plan, int64_t groups, hipStream_t s, Result& res, OutSpec& os);
 
 // ---- strategies ----------------------------------------------------------------------
+// Lanes of a wave on one group before the wave combines their rows (65 =
+// never): 32.  Combining costs a DPP reduction per op; below ~32 lanes the
+// LDS atomics' same-address serialisation is cheaper (2^28 rows, Zipf keys:
+// hash D = 2^10 s = 1.5 2.2 ms at 32 against 2.8 ms at 8; partition
+// D = 2^22 s = 1.0 2.1 ms at 32, 4.5 ms never combining).  EG_PEEL_MIN
+// overrides it for tuning.
+int default_peel_min();
+
 struct HashTuning {
     int lds_slots = 0;        // cap on LDS slots per workgroup, 0 = what fits
     int replicas = 0;         // cap on LDS table replicas, 0 = up to 64
     int lds_max_ratio = 1 << 20;  // use LDS only if group bound <= ratio * slots
-    int peel_min = 8;         // lanes on one group before the wave combines them (65 = off)
+    int peel_min = default_peel_min();  // lanes on one group before the wave combines them (65 = off)
     int block = 1024;
     int blocks_per_cu = 0;    // 0 = occupancy
     int64_t table_slots = 0;  // 0 = from the group bound

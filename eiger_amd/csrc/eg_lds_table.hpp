@@ -298,7 +298,7 @@ __device__ __forceinline__ void canon_row(const AggSpec& spec, const uint64_t (&
 template <class OPS, int KB, int NV, bool BK>
 __device__ __forceinline__ bool lds_row(LdsTable<KB, BK>& tab, bool use_lds, int r_mask, int lane, const KeyDesc& key,
                                         const AggSpec& spec, uint64_t kraw, const uint64_t (&cv)[NV],
-                                        bool valid, int probe_limit, int peel_min)
+                                        bool valid, int probe_limit, int peel_min, dev::HotTags& hot)
 {
     using KT = typename LdsKey<KB>::T;
     int slot = -1;
@@ -334,7 +334,7 @@ __device__ __forceinline__ bool lds_row(LdsTable<KB, BK>& tab, bool use_lds, int
     }
     bool done = false;
     if (use_lds && r_mask == 0 && peel_min <= 64) {
-        done = dev::wave_peel<OPS>(slot >= 0, (uint32_t)slot, spec, peel_min,
+        done = dev::wave_peel<OPS>(slot >= 0, (uint32_t)slot, hot.lds, spec, peel_min,
                                    [&](int a) { return dev::op_operand<OPS, NV>(spec, a, cv); },
                               [&](uint64_t s) { return &tab.st[(int64_t)s]; }, (int64_t)tab.T);
     }
@@ -345,12 +345,13 @@ __device__ __forceinline__ bool lds_row(LdsTable<KB, BK>& tab, bool use_lds, int
 // One row into the HBM table (called by every lane of the wave).
 template <class OPS, int KB, int NV>
 __device__ __forceinline__ void hbm_row(const GTable& gt, const KeyDesc& key, const AggSpec& spec, bool pending,
-                                        uint64_t kraw, const uint64_t (&cv)[NV], int peel_min, uint32_t& ins)
+                                        uint64_t kraw, const uint64_t (&cv)[NV], int peel_min, uint32_t& ins,
+                                        dev::HotTags& hot)
 {
     const uint64_t e64 = dev::enc64(dev::widen_key(kraw, KB, key.is_signed));
     bool done = false;
     if (peel_min <= 64)
-        done = dev::wave_peel<OPS>(pending, e64, spec, peel_min,
+        done = dev::wave_peel<OPS>(pending, e64, hot.hbm, spec, peel_min,
                                    [&](int a) { return dev::op_operand<OPS, NV>(spec, a, cv); },
                               [&](uint64_t t) {
                                   unsigned long long* g = gt_slot(gt, t, ins);

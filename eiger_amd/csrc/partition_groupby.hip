@@ -153,7 +153,7 @@ void launch_part_agg(const KeyDesc& key, const AggSpec& spec, const int64_t* per
         auto kern = part_agg_kernel<KB, VL, GATHER>;
         if (enable_dynamic_lds((const void*)kern) < lds) raise(EG_ERR_UNSUPPORTED, "partition LDS table too large");
         KernelTimer timer("partition_agg", s);
-        int probe_limit = 64, peel_min = 8;
+        int probe_limit = 64, peel_min = default_peel_min();
         // the kernel with this request's ops compiled in, once it is available
         const jit::Kind kind = GATHER ? jit::Kind::PART_AGG_GATHER : jit::Kind::PART_AGG;
         if (hipFunction_t f = jit::lookup(kind, KB, jit::vlayout_name<VL>(), spec, rows)) {

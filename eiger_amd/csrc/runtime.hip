@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <chrono>
@@ -91,6 +92,16 @@ const DeviceInfo& device_info()
     EG_HIP(hipDeviceGetAttribute(&d.warp_size, hipDeviceAttributeWarpSize, dev));
     if (d.warp_size != 64) raise(EG_ERR_UNSUPPORTED, "wave64 device required (gfx950)");
     return cache.emplace(dev, d).first->second;
+}
+
+int default_peel_min()
+{
+    static const int v = [] {
+        const char* e = std::getenv("EG_PEEL_MIN");
+        const int m = e ? std::atoi(e) : 0;
+        return (m >= 2 && m <= 65) ? m : 32;
+    }();
+    return v;
 }
 
 size_t enable_dynamic_lds(const void* kernel)
