@@ -497,6 +497,19 @@ LoadedModel& loaded_model()
     return m;
 }
 
+// Strategy from the model's outputs o[0..3] (class scores of hash / sort /
+// partition, log2 group count / 32).  A large input whose estimated group
+// count fits one LDS table goes to hash: one streaming pass, where every other
+// strategy makes radix passes (in the training and held-out traces hash was
+// the fastest for all such queries: rows >= 2^24, estimate <= 2048).
+int algo_of(const float* o, int64_t rows)
+{
+    const double card = std::exp2(std::min(62.0, std::max(0.0, (double)o[3] * 32.0)));
+    if (rows >= ((int64_t)1 << 24) && card <= 2048.0) return EG_ALGO_HASH;
+    const int best = (o[1] > o[0] && o[1] >= o[2]) ? 1 : (o[2] > o[0] && o[2] > o[1]) ? 2 : 0;
+    return best == 0 ? EG_ALGO_HASH : best == 1 ? EG_ALGO_SORT : EG_ALGO_PARTITION;
+}
+
 // Learned strategy choice for one key column; false when no model is loaded.
 bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_bound, int nops, int value_bytes,
                 hipStream_t s, int32_t* algo, int64_t* card_est)
@@ -517,8 +530,7 @@ bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_
     std::memcpy(h, m.host_out, sizeof h);
     for (float v : h)
         if (!std::isfinite(v)) return false;  // fall back to the rule
-    const int best = (h[1] > h[0] && h[1] >= h[2]) ? 1 : (h[2] > h[0] && h[2] > h[1]) ? 2 : 0;
-    *algo = best == 0 ? EG_ALGO_HASH : best == 1 ? EG_ALGO_SORT : EG_ALGO_PARTITION;
+    *algo = algo_of(h, rows);
     if (card_est) {
         const double lg = std::min(62.0, std::max(0.0, (double)h[3] * 32.0));
         *card_est = (int64_t)std::min<double>((double)rows, std::exp2(lg));
@@ -716,8 +728,7 @@ int eg_mlp_select_batch(const eg_mlp_query* queries, int64_t num, int32_t* algos
         EG_HIP(hipStreamSynchronize(s));
         for (int64_t i = 0; i < num; ++i) {
             const float* o = y.data() + (size_t)i * eg::kOut;
-            const int best = (o[1] > o[0] && o[1] >= o[2]) ? 1 : (o[2] > o[0] && o[2] > o[1]) ? 2 : 0;
-            algos[i] = best == 0 ? EG_ALGO_HASH : best == 1 ? EG_ALGO_SORT : EG_ALGO_PARTITION;
+            algos[i] = eg::algo_of(o, queries[i].rows);
             if (card_est) {
                 const double lg = std::min(62.0, std::max(0.0, (double)o[3] * 32.0));
                 card_est[i] = (int64_t)std::min<double>((double)std::max<int64_t>(queries[i].rows, 1), std::exp2(lg));

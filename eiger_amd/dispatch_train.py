@@ -134,9 +134,19 @@ def soft_targets(times: np.ndarray, tau: float) -> np.ndarray:
     return (w / w.sum(axis=1, keepdims=True)).astype(np.float32)
 
 
+def picks(y: np.ndarray, features: np.ndarray) -> np.ndarray:
+    """Strategy index (hash, sort, partition) the library picks from model
+    outputs y: the best class score, except that inputs of >= 2^24 rows whose
+    estimated group count is <= 2048 go to hash (mlp.hip algo_of)."""
+    pick = np.argmax(y[:, :3], axis=1)
+    rows = np.exp2(features[:, 0].astype(np.float64) * 32) - 1
+    card = np.exp2(np.clip(y[:, 3].astype(np.float64) * 32, 0, 62))
+    return np.where((rows >= 2.0 ** 24) & (card <= 2048), 0, pick)
+
+
 def choice_ms(params: np.ndarray, tr: dict) -> float:
     y = G.mlp_forward(params, tr["features"])
-    pick = np.argmax(y[:, :3], axis=1)
+    pick = picks(y, tr["features"])
     return float(tr["times"][np.arange(len(pick)), pick].sum())
 
 
@@ -146,7 +156,7 @@ def evaluate(params: np.ndarray, tr: dict, mlp_overhead_ms: float, rule_overhead
     x, t = tr["features"], tr["times"]
     n = len(x)
     y = G.mlp_forward(params, x)
-    pick = np.argmax(y[:, :3], axis=1)
+    pick = picks(y, x)
     best = t.min(axis=1)
     mlp_choice = float(t[np.arange(n), pick].sum())
     rule_choice = float(t[np.arange(n), tr["rule"]].sum())
