@@ -13,7 +13,7 @@
 //   2. LSD radix passes on the fmix bits move the key and every value column
 //      together (PARTITION) or the key and row ids (PARTITION_GFUR, values
 //      gathered at aggregation time);
-//   3. part_bounds_kernel finds each partition's first row;
+//   3. part_bounds_kernel finds each partition's first row (binary search);
 //   4. part_agg_kernel aggregates a unit (a partition, or a slice of a large
 //      one) in an LDS table and writes the finished groups of single-unit
 //      partitions straight to the output.  Groups of sliced partitions, and
@@ -99,17 +99,22 @@ __device__ __forceinline__ uint32_t pid_of(uint64_t raw, uint32_t mask)
     return (uint32_t)dev::bit_mix<KB>(raw) & mask;
 }
 
+// First row of every partition: after the radix passes the rows are ordered
+// by partition id, so thread p binary-searches the first row whose id is
+// >= p (offs[P] = n).  About log2(n) dependent loads per thread instead of a
+// pass over every key.
 template <int KB>
 __global__ void part_bounds_kernel(const void* keys, int64_t n, uint32_t mask, int64_t* offs)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int64_t p = pid_of<KB>(dev::load_raw(keys, KB, i), mask);
-        const int64_t pp = i == 0 ? -1 : (int64_t)pid_of<KB>(dev::load_raw(keys, KB, i - 1), mask);
-        for (int64_t q = pp + 1; q <= p; ++q) offs[q] = i;
-        if (i == n - 1)
-            for (int64_t q = p + 1; q <= (int64_t)mask + 1; ++q) offs[q] = n;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > (int64_t)mask + 1) return;
+    int64_t lo = 0, hi = n;  // invariant: rows < lo have id < p, rows >= hi have id >= p
+    while (lo < hi) {        // at most 1 + log2(n) iterations
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if ((int64_t)pid_of<KB>(dev::load_raw(keys, KB, mid), mask) < p) lo = mid + 1;
+        else hi = mid;
     }
+    offs[p] = lo;
 }
 
 // Appends the HBM table's groups after the directly written ones.
@@ -307,7 +312,7 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     std::unique_ptr<HostPhase> phase_units(new HostPhase("host_partition_units", s));
     DevBuf offs((size_t)(P + 1) * 8, s);
     {
-        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)di.num_cus * 8));
+        const int64_t grid = (P + 1 + 255) / 256;  // one thread per boundary
         KernelTimer timer("partition_bounds", s);
         if (kb == 4) hipLaunchKernelGGL(part_bounds_kernel<4>, dim3((unsigned)grid), dim3(256), 0, s, pkeys, n, pmask, offs.get<int64_t>());
         else hipLaunchKernelGGL(part_bounds_kernel<8>, dim3((unsigned)grid), dim3(256), 0, s, pkeys, n, pmask, offs.get<int64_t>());
