@@ -40,8 +40,9 @@ STRATS = {"auto": "groupby", "hash": "hash_groupby", "sort": "sort_groupby", "so
 KERNELS = ("hash_agg", "hash_compact", "hll", "key_range", "radix_hist", "radix_scan", "radix_scatter", "seg_heads",
            "segreduce", "partition_bounds", "partition_agg", "partition_compact", "fill_zero", "mlp_select")
 # timer name -> kernel function name (rocprofv3 reports the function)
-TIMER_KERNEL = {"partition_agg": "part_agg_kernel", "segreduce": "seg_reduce_kernel",
-                "partition_bounds": "part_bounds_kernel", "partition_compact": "part_compact_kernel"}
+TIMER_KERNEL = {"partition_agg": ("part_agg_kernel", "eg_jit_part_agg"), "segreduce": ("seg_reduce_kernel",),
+                "hash_agg": ("hash_agg_kernel", "eg_jit_hash_agg"),
+                "partition_bounds": ("part_bounds_kernel",), "partition_compact": ("part_compact_kernel",)}
 KERNEL_BYTES_PER_ROW = {"hash_agg": 16, "partition_agg": 16, "segreduce": 16, "radix_scatter": 32,
                         "radix_hist": 4, "seg_heads": 4, "key_range": 4, "hll": 4, "partition_bounds": 4}
 
@@ -230,7 +231,7 @@ def main():
             # measured HBM bytes per launch of this kernel on this workload
             # (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by the copy kernel)
             for kname, rec in json.load(open(tf)).get("kernels", {}).items():
-                if kname.startswith(TIMER_KERNEL.get(dom, dom + "_kernel")):
+                if kname.startswith(TIMER_KERNEL.get(dom, (dom + "_kernel",))):
                     traffic = int(rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"])
                     break
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9,
