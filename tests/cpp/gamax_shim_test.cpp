@@ -1,5 +1,6 @@
 // gamax_shim_test.cpp -- the reference's own group-by / sort / partition test
-// cases, written against the header-only gamax C++ mirror (gamax/gamax.hpp)
+// cases, written against the header-only gamax C++ mirror (gamax/gamax.hpp,
+// gamax/phys.hpp, gamax/pullops.hpp)
 // and run on the GPU through libeiger_groupby.so.  Each case follows a
 // reference test (file:line relative to src/cuda/ of the reference repo); the
 // expected values are the closed forms and fixtures those tests hold.
@@ -14,9 +15,12 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "gamax/gamax.hpp"
+#include "gamax/phys.hpp"
+#include "gamax/pullops.hpp"
 
 using namespace gamax;
 using groupby::aggregation_id;
@@ -218,6 +222,105 @@ static void test_partition_and_sort()
     pass("partition_and_sort");
 }
 
+// phys/phys_test.cu:455-476,520-548: the engine-facing wrappers.  Empty
+// inputs: group_by gives 0 rows with schema [key, SUM output]; the scalar
+// aggregate gives one row of zeros.  Keys {1,1,2,2,3}, values {10..50}:
+// three groups, SUM {30,70,50}; values {10,20,30}: SUM 60 (SUM(int32) is
+// INT64, aggregate.cu:20-53).  The caller's hint is not resolved in place.
+static void test_phys_wrappers()
+{
+    const phys::Context ctx{};
+    auto ek = one_col_table(std::vector<int32_t>{}, data_type(type_id::INT32));
+    auto ev = one_col_table(std::vector<double>{}, data_type(type_id::FLOAT64));
+    auto g0 = phys::group_by(ek->view(), ev->view(), {{0, aggregation_id::SUM}}, ctx);
+    CHECK(g0->num_rows() == 0 && g0->num_columns() == 2);
+    CHECK(g0->get_column(0).type() == data_type(type_id::INT32));
+    CHECK(g0->get_column(1).type() == data_type(type_id::FLOAT64));
+    auto s0 = phys::scalar_aggregate(ek->view(), {{0, aggregation_id::SUM}, {-1, aggregation_id::COUNT_ALL}}, ctx);
+    CHECK(s0->num_rows() == 1 && s0->num_columns() == 2);
+    CHECK(s0->get_column(0).type() == data_type(type_id::INT64));
+    CHECK(copy_column_to_host<int64_t>(s0->get_column(0).view())[0] == 0);
+    CHECK(copy_column_to_host<int64_t>(s0->get_column(1).view())[0] == 0);
+
+    auto kt = one_col_table(std::vector<int32_t>{1, 1, 2, 2, 3}, data_type(type_id::INT32));
+    auto vt = one_col_table(std::vector<int32_t>{10, 20, 30, 40, 50}, data_type(type_id::INT32));
+    groupby_hint hint;  // AUTO
+    auto g = phys::group_by(kt->view(), vt->view(), {{0, aggregation_id::SUM}}, ctx, hint);
+    CHECK(g->num_rows() == 3 && g->num_columns() == 2);
+    auto m = as_map<int32_t, int64_t>(*g, 1);
+    CHECK(m.size() == 3 && m[1] == 30 && m[2] == 70 && m[3] == 50);
+    CHECK(hint.groupby_algo == groupby::AUTO_ALGO);
+
+    auto st = one_col_table(std::vector<int32_t>{10, 20, 30}, data_type(type_id::INT32));
+    auto s = phys::scalar_aggregate(st->view(), {{0, aggregation_id::SUM}}, ctx);
+    CHECK(s->num_rows() == 1 && copy_column_to_host<int64_t>(s->get_column(0).view())[0] == 60);
+    pass("phys_wrappers");
+}
+
+// The pull-based GroupBy operator (designed here; the reference snapshot has
+// no GroupBy pullop): a scan in 100000-row chunks under a GroupByOperator
+// emitting 1000-row chunks gives the same groups as one groupby call; an
+// empty input gives no chunk.
+static void test_pullops_group_by()
+{
+    const int64_t n = 1000000;
+    std::mt19937_64 rng(5);
+    std::vector<int32_t> hk((size_t)n);
+    std::vector<int64_t> hv((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        hk[(size_t)i] = (int32_t)(rng() % 5000) - 2500;
+        hv[(size_t)i] = (int64_t)(rng() % 2000001) - 1000000;
+    }
+    std::vector<std::unique_ptr<column>> cols;
+    cols.push_back(make_column(hk, data_type(type_id::INT32)));
+    cols.push_back(make_column(hv, data_type(type_id::INT64)));
+    table input(std::move(cols));
+    std::vector<aggregation_request> reqs{{0, aggregation_id::SUM}, {-1, aggregation_id::COUNT_ALL},
+                                          {0, aggregation_id::MAX}};
+
+    pullops::TableScan scan(input.view(), 100000);
+    pullops::GroupByOperator gb(&scan, 1, reqs, groupby_hint{}, 1000);
+    gb.open();
+    std::map<int32_t, std::tuple<int64_t, int64_t, int64_t>> got;
+    int chunks = 0;
+    for (auto c = gb.next(); !c.is_end(); c = gb.next()) {
+        CHECK(c.num_items <= 1000 && c.data->num_columns() == 4);
+        auto k = copy_column_to_host<int32_t>(c.data->get_column(0).view());
+        auto s = copy_column_to_host<int64_t>(c.data->get_column(1).view());
+        auto n1 = copy_column_to_host<int64_t>(c.data->get_column(2).view());
+        auto mx = copy_column_to_host<int64_t>(c.data->get_column(3).view());
+        for (size_t i = 0; i < k.size(); ++i) {
+            CHECK(got.count(k[i]) == 0);
+            got[k[i]] = std::make_tuple(s[i], n1[i], mx[i]);
+        }
+        c.free_mem();
+        ++chunks;
+    }
+    gb.close();
+    CHECK(gb.hint_used().groupby_algo != groupby::AUTO_ALGO);
+
+    std::vector<size_type> k0{0}, v0{1};
+    groupby_hint h;
+    auto direct = groupby::groupby(input.view().select_columns(k0), input.view().select_columns(v0), reqs, h);
+    auto ds = as_map<int32_t, int64_t>(*direct, 1);
+    auto dc = as_map<int32_t, int64_t>(*direct, 2);
+    auto dm = as_map<int32_t, int64_t>(*direct, 3);
+    CHECK(got.size() == ds.size() && chunks == (int)((ds.size() + 999) / 1000));
+    for (auto const& [k, t] : got)
+        CHECK(std::get<0>(t) == ds[k] && std::get<1>(t) == dc[k] && std::get<2>(t) == dm[k]);
+
+    std::vector<std::unique_ptr<column>> ecols;
+    ecols.push_back(make_column(std::vector<int32_t>{}, data_type(type_id::INT32)));
+    ecols.push_back(make_column(std::vector<int64_t>{}, data_type(type_id::INT64)));
+    table empty(std::move(ecols));
+    pullops::TableScan escan(empty.view(), 100000);
+    pullops::GroupByOperator egb(&escan, 1, reqs);
+    egb.open();
+    CHECK(egb.next().is_end());
+    egb.close();
+    pass("pullops_group_by");
+}
+
 // groupby_hash.cu:32-40 / groupby.cu:322-323: bad calls throw std::runtime_error.
 static void test_errors()
 {
@@ -255,6 +358,8 @@ int main()
     test_avg_and_agg_count();
     test_scalar_aggregate();
     test_partition_and_sort();
+    test_phys_wrappers();
+    test_pullops_group_by();
     test_errors();
     std::printf("ALL PASSED\n");
     return g_failed;

@@ -1,4 +1,5 @@
-"""The header-only gamax C++ mirror (eiger_amd/include/gamax/gamax.hpp):
+"""The header-only gamax C++ mirror (eiger_amd/include/gamax/: gamax.hpp,
+phys.hpp, pullops.hpp):
 compiles and links against the C ABI here; the reference-style C++ test
 program (tests/cpp/gamax_shim_test.cpp) runs on the GPU."""
 import os
@@ -17,7 +18,8 @@ def _build(out_dir):
     exe = os.path.join(str(out_dir), "gamax_shim_test")
     cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
            "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "eiger_amd", "include"), SRC,
-           "-L" + LIBDIR, "-leiger_groupby", "-Wl,-rpath," + LIBDIR, "-o", exe]
+           "-L" + LIBDIR, "-leiger_groupby", "-Wl,-rpath," + LIBDIR, "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-4000:]
     return exe
