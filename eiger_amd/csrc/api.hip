@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "eg_internal.hpp"
+#include "eg_skf.hpp"
 
 namespace eg {
 const char* last_error_cstr();
@@ -304,12 +305,48 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
         if (!out) raise(EG_ERR_INVALID, "out is null");
         *out = nullptr;
         if (num_keys <= 0 || !keys) raise(EG_ERR_INVALID, "Aggregation only is not supported yet");
-        if (num_keys > 1) raise(EG_ERR_UNSUPPORTED, "only single-column group keys are supported");
         if (num_vals < 0 || (num_vals > 0 && !vals)) raise(EG_ERR_INVALID, "bad value columns");
         eg_groupby_hint local;
         eg_hint_init(&local);
         eg_groupby_hint& h = hint ? *hint : local;
         hipStream_t s = as_stream(stream);
+
+        if (num_keys > 1) {
+            // Multi-column keys: the sort and partition strategies take one key
+            // column (groupby_sort.cu:209-211, groupby.cu:287-294) unless the
+            // keys are fused (SKF, groupby.cu:396-465); hash and AUTO take them.
+            // Here every multi-column request runs on the fused key.
+            const bool strategy_entry = entry != Entry::TOP && entry != Entry::HASH;
+            const bool forced = entry == Entry::TOP && h.groupby_algo != EG_ALGO_AUTO && h.groupby_algo != EG_ALGO_HASH;
+            if (strategy_entry || (forced && !h.enable_skf))
+                raise(EG_ERR_UNSUPPORTED, "multi-column keys need the hash strategy or key fusion (enable_skf)");
+            std::vector<ViewIn> kv;
+            for (int64_t i = 0; i < num_keys; ++i) kv.push_back(view_in(keys[i]));
+            FusedKeys f;
+            if (!fuse_keys(kv.data(), num_keys, s, f))
+                raise(EG_ERR_UNSUPPORTED, "multi-column keys need more than 64 bits after range reduction");
+            eg_column_view fk{f.fused.get(), f.rows, EG_UINT64, 0, 0};
+            eg_groupby_hint h2 = h;
+            if (f.total_bits < 62 && h2.groups_upper_bound > ((int64_t)1 << f.total_bits))
+                h2.groups_upper_bound = (int64_t)1 << f.total_bits;  // SKF tightens the bound (groupby.cu:416)
+            eg_table t = nullptr;
+            const int rc = groupby_entry(entry, &fk, 1, vals, num_vals, reqs, num_reqs, &h2, stream, &t);
+            if (rc != EG_OK) raise(rc, last_error_cstr());
+            h.groupby_algo = h2.groupby_algo;
+            h.optimize_sort = h2.optimize_sort;
+            // replace the fused key column by the decoded key columns
+            std::unique_ptr<eg_table_s> tt(t);
+            const int64_t groups = tt->cols.empty() ? 0 : tt->cols[0].size;
+            std::vector<DevBuf> kc = unfuse_keys(f, tt->cols[0].data, groups, s);
+            EG_HIP(hipStreamSynchronize(s));
+            if (tt->cols[0].data) mm_free(tt->cols[0].data);
+            std::vector<eg_table_s::Col> cols;
+            for (int64_t i = 0; i < num_keys; ++i) cols.push_back(eg_table_s::Col{kc[(size_t)i].release(), groups, kv[(size_t)i].type});
+            for (size_t j = 1; j < tt->cols.size(); ++j) cols.push_back(tt->cols[j]);
+            tt->cols = std::move(cols);
+            *out = tt.release();
+            return;
+        }
 
         ViewIn k = view_in(keys[0]);
         std::vector<ViewIn> vs;

@@ -111,9 +111,13 @@ void eg_hint_init(eg_groupby_hint* hint);
  * groupby.cu:396-465).  Result columns: [key, one column per request in
  * request order, trailing COUNT if hint->agg_count] (groupby.cu:326-388), typed
  * as agg_output_data_type (aggregate.cu:20-53); num_rows = number of groups.
- * Row order is strategy-dependent and unspecified.  Single-column integer
- * keys (4 or 8 bytes); an empty input gives a 0-row table with that schema
- * (phys/phys.cu:589-611). */
+ * Row order is strategy-dependent and unspecified.  Integer keys of 4 or 8
+ * bytes; several integer key columns (up to 8) are fused into one 64-bit key
+ * after range reduction (the reference's SKF path, groupby.cu:396-465;
+ * compress/multikey_compress.hpp:11-43) -- with AUTO or HASH, or any
+ * strategy when enable_skf is set -- and fail as unsupported when the fused
+ * key would exceed 64 bits.  An empty input gives a 0-row table with that
+ * schema (phys/phys.cu:589-611). */
 int eg_groupby(const eg_column_view* keys, int64_t num_keys,
                const eg_column_view* vals, int64_t num_vals,
                const eg_agg_request* reqs, int64_t num_reqs,

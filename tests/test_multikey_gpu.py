@@ -1,0 +1,152 @@
+"""Multi-column group keys (key fusion into one 64-bit key, the reference's
+SKF path).  The reference's own multi-key fixtures (tests/groupby/
+groupby_multikeys_test.cu) under the same strategy/SKF parameter sets
+(:1475-1486), random multi-key inputs against pandas' group-by (an
+independent implementation: bit-exact for integer results, float32 SUM and
+AVG within 1e-6 relative), the error cases, and empty input."""
+import numpy as np
+import pandas as pd
+import pytest
+
+from eiger_amd import gamax as G
+
+pytestmark = pytest.mark.gpu
+
+A = G.aggregation_id
+ALGO = G.groupby_algo
+
+# groupby_multikeys_test.cu:1478-1486
+PARAMS = [("Hash_NoSKF", ALGO.HASH, False), ("Hash_SKF", ALGO.HASH, True), ("Sort_SKF", ALGO.SORT, True),
+          ("SortOpt_SKF", ALGO.SORT_OPT, True), ("SortGFUR_SKF", ALGO.SORT_GFUR, True),
+          ("Partition_SKF", ALGO.PARTITION, True), ("PartitionGFUR_SKF", ALGO.PARTITION_GFUR, True)]
+
+I32 = np.int32
+# (name, key columns, value columns, requests, expected {key tuple: values})
+FIXTURES = [
+    ("TwoIntKeyColumns_Sum",  # :54-108
+     [np.array([1, 1, 2, 2, 1], I32), np.array([10, 20, 10, 20, 10], I32)],
+     [np.array([100, 200, 300, 400, 500], I32)], [(0, A.SUM)],
+     {(1, 10): (600,), (1, 20): (200,), (2, 10): (300,), (2, 20): (400,)}),
+    ("ThreeIntKeyColumns_Sum",  # :111-171
+     [np.array([1, 1, 1, 1, 2, 2], I32), np.array([1, 1, 2, 2, 1, 1], I32), np.array([1, 2, 1, 2, 1, 2], I32)],
+     [np.array([10, 20, 30, 40, 50, 60], I32)], [(0, A.SUM)],
+     {(1, 1, 1): (10,), (1, 1, 2): (20,), (1, 2, 1): (30,), (1, 2, 2): (40,), (2, 1, 1): (50,), (2, 1, 2): (60,)}),
+    ("TwoIntKeyColumns_Max",  # :174-226
+     [np.array([1, 1, 1, 2, 2, 2], I32), np.array([1, 1, 2, 1, 2, 2], I32)],
+     [np.array([10, 50, 30, 40, 20, 60], I32)], [(0, A.MAX)],
+     {(1, 1): (50,), (1, 2): (30,), (2, 1): (40,), (2, 2): (60,)}),
+    ("MixedTypeKeys_Int32AndInt64",  # :229-282
+     [np.array([1, 1, 2, 2], I32), np.array([100, 200, 100, 200], np.int64)],
+     [np.array([10, 20, 30, 40], I32)], [(0, A.SUM)],
+     {(1, 100): (10,), (1, 200): (20,), (2, 100): (30,), (2, 200): (40,)}),
+    ("TwoKeys_MultipleAggregations",  # :285-346
+     [np.array([1, 1, 2, 2, 1], I32), np.array([1, 2, 1, 2, 1], I32)],
+     [np.array([10, 20, 30, 40, 50], I32), np.array([5, 15, 25, 35, 45], I32)], [(0, A.SUM), (1, A.SUM)],
+     {(1, 1): (60, 50), (1, 2): (20, 15), (2, 1): (30, 25), (2, 2): (40, 35)}),
+    ("SingleGroup_AllSameKeys",  # :349-390
+     [np.array([1, 1, 1, 1, 1], I32), np.array([100, 100, 100, 100, 100], I32)],
+     [np.array([10, 20, 30, 40, 50], I32)], [(0, A.SUM)],
+     {(1, 100): (150,)}),
+    ("AllUniqueKeys_EachRowIsGroup",  # :393-442
+     [np.array([1, 2, 3, 4, 5], I32), np.array([10, 20, 30, 40, 50], I32)],
+     [np.array([100, 200, 300, 400, 500], I32)], [(0, A.SUM)],
+     {(1, 10): (100,), (2, 20): (200,), (3, 30): (300,), (4, 40): (400,), (5, 50): (500,)}),
+    ("NegativeKeyValues",  # :445-491
+     [np.array([-1, -1, 1, 1], I32), np.array([-10, 10, -10, 10], I32)],
+     [np.array([100, 200, 300, 400], I32)], [(0, A.SUM)],
+     {(-1, -10): (100,), (-1, 10): (200,), (1, -10): (300,), (1, 10): (400,)}),
+    ("FirstKeyMatchesSecondDiffers",  # :494-541
+     [np.array([1, 1, 1, 1, 1, 1], I32), np.array([1, 2, 3, 1, 2, 3], I32)],
+     [np.array([10, 20, 30, 40, 50, 60], I32)], [(0, A.SUM)],
+     {(1, 1): (50,), (1, 2): (70,), (1, 3): (90,)}),
+]
+
+
+def run(keys, vals, reqs, hint=None, fn=G.groupby):
+    kcs = [G.make_column_from_host(k) for k in keys]
+    vcs = [G.make_column_from_host(v) for v in vals]
+    t = fn(kcs, vcs, reqs, hint or G.groupby_hint())
+    out = t.to_numpy()
+    t.free()
+    return out
+
+
+@pytest.mark.parametrize("pname,algo,skf", PARAMS, ids=[p[0] for p in PARAMS])
+@pytest.mark.parametrize("case", FIXTURES, ids=[f[0] for f in FIXTURES])
+def test_reference_multikey_fixture(case, pname, algo, skf):
+    name, keys, vals, reqs, expected = case
+    hint = G.groupby_hint(groupby_algo=algo, enable_skf=skf, groups_upper_bound=10)
+    out = run(keys, vals, reqs, hint)
+    nk = len(keys)
+    assert len(out) == nk + len(reqs)
+    for j, k in enumerate(keys):
+        assert out[j].dtype == k.dtype
+    got = {tuple(int(out[j][i]) for j in range(nk)): tuple(int(out[nk + r][i]) for r in range(len(reqs)))
+           for i in range(len(out[0]))}
+    assert len(got) == len(out[0]) == len(expected)
+    assert got == expected
+    if algo in (ALGO.SORT, ALGO.SORT_OPT, ALGO.SORT_GFUR):
+        # the fused key orders the tuples lexicographically
+        tuples = [tuple(int(out[j][i]) for j in range(nk)) for i in range(len(out[0]))]
+        assert tuples == sorted(tuples)
+
+
+def random_case(n, seed):
+    rng = np.random.default_rng(seed)
+    k1 = rng.integers(-50, 50, n).astype(np.int32)
+    k2 = (rng.integers(0, 40, n) * 1_000_000_007 - (1 << 40)).astype(np.int64)
+    k3 = rng.integers(0, 3, n).astype(np.uint8)
+    v = rng.integers(-(1 << 31), 1 << 31, n).astype(np.int64)
+    f = rng.random(n).astype(np.float32)
+    return [k1, k2, k3], [v, f]
+
+
+REQS = [(0, A.SUM), (-1, A.COUNT_ALL), (0, A.MIN), (0, A.MAX), (1, A.SUM), (1, A.AVG)]
+
+
+def pandas_expected(keys, vals):
+    df = pd.DataFrame({"k1": keys[0], "k2": keys[1], "k3": keys[2], "v": vals[0],
+                       "f": vals[1].astype(np.float64)})
+    g = df.groupby(["k1", "k2", "k3"], sort=True)
+    e = pd.DataFrame({"sum": g["v"].sum(), "count": g["v"].count(), "min": g["v"].min(), "max": g["v"].max(),
+                      "fsum": g["f"].sum(), "favg": g["f"].mean()}).reset_index()
+    return e
+
+
+@pytest.mark.parametrize("algo,skf", [(ALGO.AUTO_ALGO, False), (ALGO.HASH, False), (ALGO.SORT, True),
+                                      (ALGO.PARTITION, True), (ALGO.PARTITION_GFUR, True)])
+def test_random_multikey_against_pandas(algo, skf):
+    n = 1 << 18
+    keys, vals = random_case(n, seed=int(algo) * 7 + 1)
+    out = run(keys, vals, REQS, G.groupby_hint(groupby_algo=algo, enable_skf=skf))
+    exp = pandas_expected(keys, vals)
+    order = np.lexsort((out[2], out[1], out[0]))
+    out = [c[order] for c in out]
+    assert len(out[0]) == len(exp)
+    for j, name in enumerate(["k1", "k2", "k3"]):
+        assert out[j].dtype == keys[j].dtype
+        np.testing.assert_array_equal(out[j], exp[name].to_numpy())
+    np.testing.assert_array_equal(out[3], exp["sum"].to_numpy())
+    np.testing.assert_array_equal(out[4], exp["count"].to_numpy())
+    np.testing.assert_array_equal(out[5], exp["min"].to_numpy())
+    np.testing.assert_array_equal(out[6], exp["max"].to_numpy())
+    np.testing.assert_allclose(out[7], exp["fsum"].to_numpy(), rtol=1e-6, atol=0)
+    np.testing.assert_allclose(out[8], exp["favg"].to_numpy(), rtol=1e-6, atol=0)
+
+
+def test_multikey_errors_and_empty_input():
+    keys, vals = random_case(1000, seed=3)
+    # the sort and partition strategy entry points take one key column
+    for fn in (G.sort_groupby, G.partition_groupby):
+        with pytest.raises(G.GamaxError):
+            run(keys, vals, [(0, A.SUM)], fn=fn)
+    # a forced non-hash strategy without key fusion (groupby.cu:287-294)
+    with pytest.raises(G.GamaxError):
+        run(keys, vals, [(0, A.SUM)], G.groupby_hint(groupby_algo=ALGO.SORT))
+    # two full-range int64 columns need 128 bits
+    wide = [np.array([np.iinfo(np.int64).min, np.iinfo(np.int64).max] * 4, np.int64)] * 2
+    with pytest.raises(G.GamaxError):
+        run(wide, [np.arange(8, dtype=np.int64)], [(0, A.SUM)])
+    empty = run([k[:0] for k in keys], [v[:0] for v in vals], [(0, A.SUM), (-1, A.COUNT_ALL)])
+    assert len(empty) == 5 and all(len(c) == 0 for c in empty)
+    assert [c.dtype for c in empty[:3]] == [np.int32, np.int64, np.uint8]
