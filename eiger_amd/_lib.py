@@ -89,6 +89,8 @@ def _proto(L):
     for name in ("eg_groupby", "eg_groupby_hash", "eg_groupby_sort", "eg_groupby_sort_gfur",
                  "eg_groupby_partition", "eg_groupby_partition_gfur"):
         f(name, C.c_int, gb)
+    f("eg_distinct_hash", C.c_int, [PCV, i64, PH, vp, PT])
+    f("eg_distinct_sort", C.c_int, [PCV, i64, PH, vp, PT])
     f("eg_aggregate", C.c_int, [PCV, i64, PAR, i64, vp, PT])
     f("eg_agg_output_type", C.c_int, [i32, i32, C.POINTER(i32)])
     f("eg_table_num_columns", i64, [vp])

@@ -483,6 +483,52 @@ int eg_groupby_partition_gfur(const eg_column_view* keys, int64_t num_keys,
     return groupby_entry(Entry::PARTITION_GFUR, keys, num_keys, vals, num_vals, reqs, num_reqs, hint, stream, out);
 }
 
+// Distinct key rows: the group-by with one COUNT_ALL request, count dropped.
+static int distinct_entry(bool sort, const eg_column_view* keys, int64_t num_keys, eg_groupby_hint* hint,
+                          void* stream, eg_table* out)
+{
+    if (!out) return eg::guard([] { eg::raise(EG_ERR_INVALID, "out is null"); });
+    *out = nullptr;
+    eg_groupby_hint local;
+    eg_hint_init(&local);
+    eg_groupby_hint& h = hint ? *hint : local;
+    const eg_agg_request cnt{-1, EG_AGG_COUNT_ALL, 0};
+    const bool agg_count = h.agg_count != 0;
+    h.agg_count = 0;
+    int rc;
+    if (sort && num_keys > 1) {
+        // multi-column keys only through key fusion (groupby_sort.hpp:39-44)
+        if (!h.enable_skf)
+            return eg::guard([] { eg::raise(EG_ERR_UNSUPPORTED, "sort::distinct: multi-column keys need enable_skf"); });
+        const int32_t algo = h.groupby_algo;
+        h.groupby_algo = EG_ALGO_SORT;
+        rc = groupby_entry(Entry::TOP, keys, num_keys, nullptr, 0, &cnt, 1, &h, stream, out);
+        h.groupby_algo = algo;
+    } else {
+        rc = groupby_entry(sort ? Entry::SORT : Entry::HASH, keys, num_keys, nullptr, 0, &cnt, 1, &h, stream, out);
+    }
+    h.agg_count = agg_count ? 1 : 0;
+    if (rc != EG_OK) return rc;
+    return eg::guard([&] {
+        eg_table t = *out;
+        if (t->cols.size() != (size_t)num_keys + 1) eg::raise(EG_ERR_DEVICE, "distinct: unexpected result shape");
+        if (t->cols.back().data) eg::mm_free(t->cols.back().data);
+        t->cols.pop_back();
+    });
+}
+
+int eg_distinct_hash(const eg_column_view* keys, int64_t num_keys, eg_groupby_hint* hint, void* stream,
+                     eg_table* out)
+{
+    return distinct_entry(false, keys, num_keys, hint, stream, out);
+}
+
+int eg_distinct_sort(const eg_column_view* keys, int64_t num_keys, eg_groupby_hint* hint, void* stream,
+                     eg_table* out)
+{
+    return distinct_entry(true, keys, num_keys, hint, stream, out);
+}
+
 int eg_aggregate(const eg_column_view* cols, int64_t num_cols, const eg_agg_request* reqs,
                  int64_t num_reqs, void* stream, eg_table* out)
 {

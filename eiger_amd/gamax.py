@@ -281,6 +281,29 @@ def _groupby_call(fn_name, key, val, requests, hint, stream):
     return _take_table(out)
 
 
+def _distinct_call(fn_name, key, hint, stream):
+    hint = hint if hint is not None else groupby_hint()
+    keys = _tv(key)
+    h = hint._to_c()
+    out = C.c_void_p()
+    rc = getattr(lib(), fn_name)(_views(keys), len(keys), C.byref(h), _stream(stream), C.byref(out))
+    hint._from_c(h)
+    _check(rc)
+    return _take_table(out)
+
+
+def hash_distinct(key, hint: Optional[groupby_hint] = None, stream=None) -> table:
+    """gamax::groupby::hash::distinct (groupby_hash.hpp:24-32): one row per
+    unique key (tuple); row order unspecified."""
+    return _distinct_call("eg_distinct_hash", key, hint, stream)
+
+
+def sort_distinct(key, hint: Optional[groupby_hint] = None, stream=None) -> table:
+    """gamax::groupby::sort::distinct (groupby_sort.hpp:31-48): one key
+    column, or several with hint.enable_skf; ascending key order."""
+    return _distinct_call("eg_distinct_sort", key, hint, stream)
+
+
 def groupby(key, val, agg_requests, hint: Optional[groupby_hint] = None, stream=None) -> table:
     """gamax::groupby::groupby (operators/groupby/groupby.hpp:44-49)."""
     return _groupby_call("eg_groupby", key, val, agg_requests, hint, stream)

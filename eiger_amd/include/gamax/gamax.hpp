@@ -312,6 +312,19 @@ inline std::unique_ptr<table> call(entry_fn fn, table_view key, table_view val,
     hint.groups_upper_bound = h.groups_upper_bound;
     return gamax::detail::adopt(out);
 }
+
+using distinct_fn = int (*)(const eg_column_view*, int64_t, eg_groupby_hint*, void*, eg_table*);
+
+inline std::unique_ptr<table> call_distinct(distinct_fn fn, table_view key, groupby_hint& hint, hipStream_t stream)
+{
+    auto k = key.abi();
+    eg_groupby_hint h = to_abi(hint);
+    eg_table out = nullptr;
+    gamax::detail::check(fn(k.data(), (int64_t)k.size(), &h, stream, &out));
+    hint.groupby_algo = (gamax::groupby::groupby_algo)h.groupby_algo;
+    hint.groups_upper_bound = h.groups_upper_bound;
+    return gamax::detail::adopt(out);
+}
 }  // namespace detail
 
 // operators/groupby/groupby.hpp:44-49
@@ -328,6 +341,11 @@ inline std::unique_ptr<table> groupby(table_view key, table_view val, std::vecto
 {
     return detail::call(eg_groupby_hash, key, val, agg_requests, hint, stream);
 }
+// operators/groupby/groupby_hash.hpp:24-32
+inline std::unique_ptr<table> distinct(table_view key, groupby_hint& hint, hipStream_t stream = nullptr)
+{
+    return detail::call_distinct(eg_distinct_hash, key, hint, stream);
+}
 }  // namespace hash
 
 namespace sort {
@@ -342,6 +360,11 @@ inline std::unique_ptr<table> groupby_gfur(table_view key, table_view val,
                                            hipStream_t stream = nullptr)
 {
     return detail::call(eg_groupby_sort_gfur, key, val, agg_requests, hint, stream);
+}
+// operators/groupby/groupby_sort.hpp:31-48
+inline std::unique_ptr<table> distinct(table_view key, groupby_hint& hint, hipStream_t stream = nullptr)
+{
+    return detail::call_distinct(eg_distinct_sort, key, hint, stream);
 }
 }  // namespace sort
 

@@ -1,7 +1,7 @@
 // phys.hpp -- the engine-facing aggregation layer over the gamax mirror:
-// gamax::phys::group_by and gamax::phys::scalar_aggregate, the functions the
-// reference's executor calls for a GroupBy / Aggregate step
-// (phys/phys.hpp:59-69,213-232, phys/phys.cu:535-621).  Header-only; needs
+// gamax::phys::group_by, scalar_aggregate and distinct, the functions the
+// reference's executor calls for a GroupBy / Aggregate / Distinct step
+// (phys/phys.hpp:59-69,213-240, phys/phys.cu:535-663).  Header-only; needs
 // gamax.hpp and libeiger_groupby.so.
 //
 // Behaviour kept from the reference:
@@ -86,6 +86,31 @@ inline table_ptr group_by(table_view keys, table_view values, std::vector<AggSpe
     auto reqs = detail::requests_of(specs);
     auto hint = hint_in;
     return gamax::groupby::groupby(keys, values, reqs, hint, ctx.stream);
+}
+
+// Distinct rows of `keys` (phys.cu:622-663): HASH / AUTO go to
+// hash::distinct, the sort family to sort::distinct, PARTITION variants
+// throw; 0 rows give 0 rows with the key schema.
+inline table_ptr distinct(table_view keys, Context const& ctx, gamax::groupby::groupby_hint const& hint_in = {})
+{
+    if (keys.num_rows() == 0) {
+        std::vector<column_ptr> cols;
+        for (size_type i = 0; i < keys.num_columns(); ++i)
+            cols.push_back(std::make_unique<column>(keys.get_column(i).type(), 0, false, ctx.stream));
+        return std::make_unique<table>(std::move(cols));
+    }
+    auto hint = hint_in;
+    switch (hint.groupby_algo) {
+    case gamax::groupby::HASH:
+    case gamax::groupby::AUTO_ALGO: return gamax::groupby::hash::distinct(keys, hint, ctx.stream);
+    case gamax::groupby::SORT:
+    case gamax::groupby::SORT_OPT:
+    case gamax::groupby::SORT_GFUR: return gamax::groupby::sort::distinct(keys, hint, ctx.stream);
+    case gamax::groupby::PARTITION:
+    case gamax::groupby::PARTITION_GFUR:
+        throw std::runtime_error("phys::distinct: PARTITION variants are not supported; use HASH or SORT");
+    default: throw std::runtime_error("phys::distinct: unknown groupby_algo");
+    }
 }
 
 }  // namespace phys

@@ -151,6 +151,18 @@ int eg_groupby_partition_gfur(const eg_column_view* keys, int64_t num_keys,
                               const eg_agg_request* reqs, int64_t num_reqs,
                               eg_groupby_hint* hint, void* stream, eg_table* out);
 
+/* Distinct rows of the key columns, one per unique key (or key tuple):
+ *   hash::distinct  operators/groupby/groupby_hash.hpp:24-32 (any key
+ *                   shape the group-by takes; first-encounter order is not
+ *                   kept: row order is unspecified)
+ *   sort::distinct  operators/groupby/groupby_sort.hpp:31-48 (one key
+ *                   column, or several with hint->enable_skf; ascending)
+ * The result has exactly the key columns. */
+int eg_distinct_hash(const eg_column_view* keys, int64_t num_keys,
+                     eg_groupby_hint* hint, void* stream, eg_table* out);
+int eg_distinct_sort(const eg_column_view* keys, int64_t num_keys,
+                     eg_groupby_hint* hint, void* stream, eg_table* out);
+
 /* gamax::groupby::aggregate::aggregate (aggregation.hpp:34-39,
  * aggregate.cu:131-165): one row, one column per request. */
 int eg_aggregate(const eg_column_view* cols, int64_t num_cols,

@@ -254,6 +254,29 @@ static void test_phys_wrappers()
     auto st = one_col_table(std::vector<int32_t>{10, 20, 30}, data_type(type_id::INT32));
     auto s = phys::scalar_aggregate(st->view(), {{0, aggregation_id::SUM}}, ctx);
     CHECK(s->num_rows() == 1 && copy_column_to_host<int64_t>(s->get_column(0).view())[0] == 60);
+
+    // phys::distinct (phys.cu:622-663): keys {1,1,2,2,3} -> {1,2,3} by hash and
+    // by sort (ascending); PARTITION throws; empty input keeps the schema.
+    groupby_hint hh;
+    auto dh = phys::distinct(kt->view(), ctx, hh);
+    auto vh = copy_column_to_host<int32_t>(dh->get_column(0).view());
+    std::sort(vh.begin(), vh.end());
+    CHECK(dh->num_columns() == 1 && vh == (std::vector<int32_t>{1, 2, 3}));
+    groupby_hint hs;
+    hs.groupby_algo = groupby::SORT;
+    auto ds = phys::distinct(kt->view(), ctx, hs);
+    CHECK(copy_column_to_host<int32_t>(ds->get_column(0).view()) == (std::vector<int32_t>{1, 2, 3}));
+    groupby_hint hp;
+    hp.groupby_algo = groupby::PARTITION;
+    bool threw = false;
+    try {
+        phys::distinct(kt->view(), ctx, hp);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    CHECK(threw);
+    auto de = phys::distinct(ek->view(), ctx);
+    CHECK(de->num_rows() == 0 && de->num_columns() == 1 && de->get_column(0).type() == data_type(type_id::INT32));
     pass("phys_wrappers");
 }
 

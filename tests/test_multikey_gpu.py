@@ -150,3 +150,38 @@ def test_multikey_errors_and_empty_input():
     empty = run([k[:0] for k in keys], [v[:0] for v in vals], [(0, A.SUM), (-1, A.COUNT_ALL)])
     assert len(empty) == 5 and all(len(c) == 0 for c in empty)
     assert [c.dtype for c in empty[:3]] == [np.int32, np.int64, np.uint8]
+
+
+def _rows(out):
+    return sorted(zip(*[c.tolist() for c in out]))
+
+
+@pytest.mark.parametrize("fn,skf", [(G.hash_distinct, False), (G.sort_distinct, False), (G.sort_distinct, True)])
+def test_distinct_single_and_multi_column(fn, skf):
+    """groupby_hash.hpp:24-32 / groupby_sort.hpp:31-48: one row per unique key
+    (tuple), exactly the key columns; sort order is ascending.  Checked
+    against numpy's unique."""
+    keys, _ = random_case(1 << 17, seed=21)
+    single = [keys[1]]
+    out = fn([G.make_column_from_host(k) for k in single], G.groupby_hint(enable_skf=skf)).to_numpy()
+    assert len(out) == 1 and out[0].dtype == np.int64
+    np.testing.assert_array_equal(np.sort(out[0]), np.unique(keys[1]))
+    if fn is G.sort_distinct:
+        assert np.all(np.diff(out[0]) > 0)
+    multi = [G.make_column_from_host(k) for k in keys]
+    if fn is G.sort_distinct and not skf:
+        with pytest.raises(G.GamaxError):
+            fn(multi, G.groupby_hint())
+        return
+    out = fn(multi, G.groupby_hint(enable_skf=skf)).to_numpy()
+    assert [c.dtype for c in out] == [np.int32, np.int64, np.uint8]
+    exp = sorted(set(zip(keys[0].tolist(), keys[1].tolist(), keys[2].tolist())))
+    assert _rows(out) == exp
+    if fn is G.sort_distinct:
+        assert list(zip(*[c.tolist() for c in out])) == exp  # ascending tuples
+
+
+def test_distinct_empty_input():
+    for fn in (G.hash_distinct, G.sort_distinct):
+        out = fn([G.make_column_from_host(np.zeros(0, np.int32))]).to_numpy()
+        assert len(out) == 1 and len(out[0]) == 0 and out[0].dtype == np.int32
