@@ -111,6 +111,8 @@ def _proto(L):
     f("eg_partition_column", C.c_int, [CV, i32, i32, vp, sz, vp, PCV])
     f("eg_arg_partition", C.c_int, [CV, i32, i32, vp, sz, vp, PCV, PCV])
     f("eg_compute_partition_offsets", C.c_int, [CV, i32, i32, vp, PCV])
+    f("eg_dictionary_encode_with_dict", C.c_int, [CV, CV, i32, vp, PCV])
+    f("eg_dictionary_encode", C.c_int, [CV, i32, i64, i32, vp, PCV, C.POINTER(i64), PCV])
     f("eg_compute_ndv_approx", C.c_int, [CV, vp, C.POINTER(i64)])
     f("eg_auto_select_algo", C.c_int, [PCV, i64, PH, vp])
     f("eg_kernel_timing_enable", C.c_int, [i32])

@@ -1,0 +1,373 @@
+// dict.hip -- dictionary encoding of an integer column
+// (gamax::compress::dictionary_encode, compress/dictionary_encode.hpp:12-37):
+//   * with a given dictionary (dictionary_encode_dict_ready.cu:15-80): each
+//     row gets the position of its key in the dictionary column, INT32, or
+//     INT32_MIN (the reference's empty-value sentinel, types/sentinel.hpp:10-16)
+//     when the key is not in it;
+//   * self-built (dictionary_encode.cu): each row gets a dense id in
+//     [0, number of distinct keys), any integral output type; with_order
+//     numbers the distinct keys in ascending key order.  SORT_OPT's
+//     re-encoding of a key column against its sorted unique keys
+//     (groupby_sort.cu:394-413) is this operation.
+//
+// The dictionary goes into an open-addressing table of 2^c >= 2 m slots
+// (m dictionary entries): slot value = dictionary position + 1 (0 = empty,
+// so a zero-fill clears it), claimed by compare-and-swap; a duplicate key
+// keeps its smallest position (atomicMin), so results are deterministic.  A
+// second pass stores each slot's key beside it.  The lookup streams the column
+// once (read key, write id): HBM-bound, 4 + 4 bytes per int32 row.  Tables of
+// up to 8192 slots are copied into each workgroup's LDS; larger ones are probed
+// in HBM (they stay L2-resident up to a few MB).  Unlike the reference's cuco
+// map (INT_MIN reserved as the empty key), every key value is valid.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+
+#include "eg_device.hpp"
+#include "eg_internal.hpp"
+
+namespace eg {
+namespace {
+
+constexpr int kDictBlock = 1024;
+constexpr int kLdsSlotsMax = 8192;  // 32 KB of positions + up to 64 KB of keys
+constexpr int64_t kMaxDict = (int64_t)1 << 30;
+
+__device__ __forceinline__ uint32_t dict_home(uint64_t raw, int cap_log2)
+{
+    return (uint32_t)(dev::fmix64(raw) >> (64 - cap_log2));
+}
+
+__global__ __launch_bounds__(256) void dict_build_kernel(const void* dict, int kb, int64_t m, uint32_t* slots,
+                                                         int cap_log2)
+{
+    const uint32_t mask = (1u << cap_log2) - 1u;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const uint64_t k = dev::load_raw(dict, kb, i);
+        const uint32_t v = (uint32_t)i + 1u;
+        uint32_t h = dict_home(k, cap_log2);
+        // at most m of the 2^cap_log2 >= 2m slots are ever taken, so an empty
+        // slot is met within the bound
+        for (uint32_t p = 0; p <= mask; ++p) {
+            const uint32_t old = atomicCAS(&slots[h], 0u, v);
+            if (old == 0u) break;
+            if (dev::load_raw(dict, kb, (int64_t)old - 1) == k) {
+                atomicMin(&slots[h], v);
+                break;
+            }
+            h = (h + 1u) & mask;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void dict_slot_keys_kernel(const void* dict, int kb, const uint32_t* slots,
+                                                             int64_t cap, uint64_t* skeys)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += stride) {
+        const uint32_t v = slots[s];
+        skeys[s] = v ? dev::load_raw(dict, kb, (int64_t)v - 1) : 0ull;
+    }
+}
+
+template <int KB>
+using SlotKey = typename std::conditional<(KB > 4), uint64_t, uint32_t>::type;
+
+template <int KB>
+__device__ __forceinline__ uint64_t load_key(const void* p, int64_t i)
+{
+    if constexpr (KB == 1) return ((const uint8_t*)p)[i];
+    else if constexpr (KB == 2) return ((const uint16_t*)p)[i];
+    else if constexpr (KB == 4) return ((const uint32_t*)p)[i];
+    else return ((const uint64_t*)p)[i];
+}
+
+template <int OB>
+__device__ __forceinline__ void store_id(void* p, int64_t i, uint64_t bits)
+{
+    if constexpr (OB == 1) ((uint8_t*)p)[i] = (uint8_t)bits;
+    else if constexpr (OB == 2) ((uint16_t*)p)[i] = (uint16_t)bits;
+    else if constexpr (OB == 4) ((uint32_t*)p)[i] = (uint32_t)bits;
+    else ((uint64_t*)p)[i] = bits;
+}
+
+// Dictionary position of k, or `miss`.
+template <typename KT>
+__device__ __forceinline__ uint64_t probe(const uint32_t* sidx, const KT* skey, uint64_t k, int cap_log2,
+                                          uint64_t miss)
+{
+    const uint32_t mask = (1u << cap_log2) - 1u;
+    uint32_t h = dict_home(k, cap_log2);
+    for (uint32_t p = 0; p <= mask; ++p) {
+        const uint32_t v = sidx[h];
+        if (v == 0u) break;
+        if ((uint64_t)skey[h] == k) return (uint64_t)(v - 1u);
+        h = (h + 1u) & mask;
+    }
+    return miss;
+}
+
+template <int KB, int OB, bool LDS>
+__global__ __launch_bounds__(kDictBlock) void dict_find_kernel(const void* __restrict__ keys, int64_t n,
+                                                               const uint32_t* __restrict__ slots,
+                                                               const uint64_t* __restrict__ skeys, int cap_log2,
+                                                               uint64_t miss, void* __restrict__ out)
+{
+    using KT = SlotKey<KB>;
+    extern __shared__ __align__(16) unsigned char dict_smem[];
+    const uint32_t* sidx = slots;
+    const KT* skey = nullptr;
+    if constexpr (LDS) {
+        const int cap = 1 << cap_log2;
+        uint32_t* li = (uint32_t*)dict_smem;
+        KT* lk = (KT*)(dict_smem + (size_t)cap * 4);
+        for (int s = threadIdx.x; s < cap; s += blockDim.x) {
+            li[s] = slots[s];
+            lk[s] = (KT)skeys[s];
+        }
+        __syncthreads();
+        sidx = li;
+        skey = lk;
+    } else {
+        static_assert(sizeof(KT) <= 8, "slot keys");
+        if constexpr (sizeof(KT) == 8) skey = (const KT*)skeys;
+    }
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if constexpr (!LDS && sizeof(KT) != 8) {
+        // HBM slot keys are 64-bit words
+        for (; i < n; i += stride) store_id<OB>(out, i, probe(sidx, skeys, load_key<KB>(keys, i), cap_log2, miss));
+        return;
+    } else {
+        for (; i + 3 * stride < n; i += 4 * stride) {
+            uint64_t k[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) k[u] = load_key<KB>(keys, i + u * stride);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) store_id<OB>(out, i + u * stride, probe(sidx, skey, k[u], cap_log2, miss));
+        }
+        for (; i < n; i += stride) store_id<OB>(out, i, probe(sidx, skey, load_key<KB>(keys, i), cap_log2, miss));
+    }
+}
+
+template <int KB, int OB, bool LDS>
+void launch_find(const void* keys, int64_t n, const uint32_t* slots, const uint64_t* skeys, int cap_log2,
+                 uint64_t miss, void* out, hipStream_t s)
+{
+    const void* kern = (const void*)dict_find_kernel<KB, OB, LDS>;
+    size_t lds = 0;
+    if (LDS) {
+        lds = ((size_t)4 + sizeof(SlotKey<KB>)) << cap_log2;
+        if (lds > enable_dynamic_lds(kern)) raise(EG_ERR_DEVICE, "dictionary table exceeds LDS");
+    }
+    int per_cu = 0;
+    EG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDictBlock, lds));
+    per_cu = std::max(per_cu, 1);
+    const int64_t grid =
+        std::max<int64_t>(1, std::min<int64_t>((n + kDictBlock - 1) / kDictBlock, (int64_t)device_info().num_cus * per_cu));
+    hipLaunchKernelGGL((dict_find_kernel<KB, OB, LDS>), dim3((unsigned)grid), dim3(kDictBlock), lds, s, keys, n,
+                       slots, skeys, cap_log2, miss, out);
+    EG_LAUNCH_CHECK();
+}
+
+template <int KB, bool LDS>
+void launch_find_ob(int ob, const void* keys, int64_t n, const uint32_t* slots, const uint64_t* skeys,
+                    int cap_log2, uint64_t miss, void* out, hipStream_t s)
+{
+    switch (ob) {
+    case 1: launch_find<KB, 1, LDS>(keys, n, slots, skeys, cap_log2, miss, out, s); break;
+    case 2: launch_find<KB, 2, LDS>(keys, n, slots, skeys, cap_log2, miss, out, s); break;
+    case 4: launch_find<KB, 4, LDS>(keys, n, slots, skeys, cap_log2, miss, out, s); break;
+    default: launch_find<KB, 8, LDS>(keys, n, slots, skeys, cap_log2, miss, out, s); break;
+    }
+}
+
+template <bool LDS>
+void launch_find_kb(int kb, int ob, const void* keys, int64_t n, const uint32_t* slots, const uint64_t* skeys,
+                    int cap_log2, uint64_t miss, void* out, hipStream_t s)
+{
+    switch (kb) {
+    case 1: launch_find_ob<1, LDS>(ob, keys, n, slots, skeys, cap_log2, miss, out, s); break;
+    case 2: launch_find_ob<2, LDS>(ob, keys, n, slots, skeys, cap_log2, miss, out, s); break;
+    case 4: launch_find_ob<4, LDS>(ob, keys, n, slots, skeys, cap_log2, miss, out, s); break;
+    default: launch_find_ob<8, LDS>(ob, keys, n, slots, skeys, cap_log2, miss, out, s); break;
+    }
+}
+
+// 1- and 2-byte keys are widened to INT32 for the distinct-key pass (the
+// group-by takes 4- and 8-byte keys); widening keeps order and equality.
+__global__ __launch_bounds__(256) void widen_kernel(const void* in, int kb, int sign, int64_t n, int32_t* out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t r = (uint32_t)dev::load_raw(in, kb, i);
+        const int sh = 32 - 8 * kb;
+        out[i] = sign ? ((int32_t)(r << sh) >> sh) : (int32_t)r;
+    }
+}
+
+__global__ __launch_bounds__(256) void narrow_kernel(const int32_t* in, int kb, int64_t n, void* out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dev::store_bytes(out, kb, i, (uint64_t)(uint32_t)in[i]);
+}
+
+int64_t small_grid(int64_t n)
+{
+    return std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)device_info().num_cus * 8));
+}
+
+const void* view_ptr(const eg_column_view& c, int bytes)
+{
+    if (c.offset < 0 || c.size < 0) raise(EG_ERR_INVALID, "bad column view");
+    if (c.size > 0 && !c.data) raise(EG_ERR_INVALID, "null column data");
+    return c.data ? (const char*)c.data + c.offset * bytes : nullptr;
+}
+
+void check_key_type(int32_t t)
+{
+    if (!is_int(t) && t != EG_BOOL8)
+        raise(EG_ERR_UNSUPPORTED, "Dictionary encoding is not supported for non-integral type.");
+}
+
+}  // namespace
+
+// Positions of column's keys in dict (n rows of `out_type`, `miss` where a key
+// is absent).  Both columns have the same integer type.
+DevBuf dict_encode(const void* keys, int64_t n, const void* dict, int64_t m, int kb, int32_t out_type, uint64_t miss,
+                   hipStream_t s)
+{
+    const int ob = type_size(out_type);
+    DevBuf out((size_t)std::max<int64_t>(n, 1) * ob, s);
+    if (n == 0) return out;
+    if (m > kMaxDict) raise(EG_ERR_UNSUPPORTED, "dictionary larger than 2^30 entries");
+    // an empty dictionary gives an empty 64-slot table: every key is absent
+    int cap_log2 = 6;
+    while (((int64_t)1 << cap_log2) < 2 * m) ++cap_log2;
+    const int64_t cap = (int64_t)1 << cap_log2;
+    DevBuf slots((size_t)cap * 4, s);
+    fill_zero(slots.get(), (size_t)cap * 4, s);
+    DevBuf skeys((size_t)cap * 8, s);
+    if (m > 0) {
+        hipLaunchKernelGGL(dict_build_kernel, dim3((unsigned)small_grid(m)), dim3(256), 0, s, dict, kb, m,
+                           slots.get<uint32_t>(), cap_log2);
+        EG_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(dict_slot_keys_kernel, dim3((unsigned)small_grid(cap)), dim3(256), 0, s, dict, kb,
+                       (const uint32_t*)slots.get<uint32_t>(), cap, skeys.get<uint64_t>());
+    EG_LAUNCH_CHECK();
+    if (cap <= kLdsSlotsMax)
+        launch_find_kb<true>(kb, ob, keys, n, slots.get<uint32_t>(), skeys.get<uint64_t>(), cap_log2, miss, out.get(), s);
+    else
+        launch_find_kb<false>(kb, ob, keys, n, slots.get<uint32_t>(), skeys.get<uint64_t>(), cap_log2, miss, out.get(),
+                              s);
+    EG_HIP(hipStreamSynchronize(s));  // the tables are freed on return
+    return out;
+}
+
+}  // namespace eg
+
+using namespace eg;
+
+int eg_dictionary_encode_with_dict(eg_column_view column, eg_column_view dict, int32_t encode_type, void* stream,
+                                   eg_column_view* out)
+{
+    return guard([&] {
+        if (!out) raise(EG_ERR_INVALID, "null output");
+        if (encode_type != EG_INT32) raise(EG_ERR_UNSUPPORTED, "Unsupported encoded type");
+        check_key_type(column.type);
+        if (dict.type != column.type) raise(EG_ERR_INVALID, "dictionary and column types differ");
+        const int kb = type_size(column.type);
+        const void* keys = view_ptr(column, kb);
+        const void* d = view_ptr(dict, kb);
+        hipStream_t s = (hipStream_t)stream;
+        DevBuf enc = dict_encode(keys, column.size, d, dict.size, kb, EG_INT32, (uint64_t)(uint32_t)INT32_MIN, s);
+        out->data = enc.release();
+        out->size = column.size;
+        out->type = EG_INT32;
+        out->reserved = 0;
+        out->offset = 0;
+    });
+}
+
+int eg_dictionary_encode(eg_column_view column, int32_t encode_type, int64_t dictionary_size_hint, int32_t with_order,
+                         void* stream, eg_column_view* out, int64_t* out_dict_size, eg_column_view* out_dict)
+{
+    eg_table t = nullptr;
+    const int rc = guard([&] {
+        if (!out || !out_dict_size) raise(EG_ERR_INVALID, "null output");
+        if (!is_int(encode_type)) raise(EG_ERR_UNSUPPORTED, "Encoded type must be integral.");
+        check_key_type(column.type);
+        int kb = type_size(column.type);
+        const void* keys = view_ptr(column, kb);
+        hipStream_t s = (hipStream_t)stream;
+        const int orig_kb = kb;
+        eg_column_view src = column;
+        DevBuf wide;
+        if (kb < 4) {
+            wide = DevBuf((size_t)std::max<int64_t>(column.size, 1) * 4, s);
+            if (column.size > 0) {
+                hipLaunchKernelGGL(widen_kernel, dim3((unsigned)small_grid(column.size)), dim3(256), 0, s, keys, kb,
+                                   is_signed_int(column.type) ? 1 : 0, column.size, wide.get<int32_t>());
+                EG_LAUNCH_CHECK();
+            }
+            keys = wide.get();
+            kb = 4;
+            src.data = wide.get();
+            src.offset = 0;
+            src.type = EG_INT32;
+        }
+        // the distinct keys: ascending (sort) for with_order, any order (hash) otherwise
+        eg_groupby_hint h;
+        eg_hint_init(&h);
+        if (dictionary_size_hint > 0) h.groups_upper_bound = dictionary_size_hint;
+        const int r = with_order ? eg_distinct_sort(&src, 1, &h, stream, &t) : eg_distinct_hash(&src, 1, &h, stream, &t);
+        if (r != EG_OK) raise(r, eg_last_error());
+        eg_column_view dv{};
+        if (eg_table_release_column(t, 0, &dv) != EG_OK) raise(EG_ERR_DEVICE, "dictionary_encode: no key column");
+        eg_table_free(t);
+        t = nullptr;
+        struct FreeOnExit {
+            void* p;
+            ~FreeOnExit() { if (p) eg_free(p); }
+        } dict_owner{const_cast<void*>(dv.data)};
+        const int ob = type_size(encode_type);
+        if (ob < 8) {
+            const int vb = 8 * ob - (is_signed_int(encode_type) ? 1 : 0);
+            if (dv.size > ((int64_t)1 << vb)) raise(EG_ERR_INVALID, "dictionary does not fit the encoded type");
+        }
+        DevBuf enc = dict_encode(keys, column.size, dv.data, dv.size, kb, encode_type, 0ull, s);
+        out->data = enc.release();
+        out->size = column.size;
+        out->type = encode_type;
+        out->reserved = 0;
+        out->offset = 0;
+        *out_dict_size = dv.size;
+        if (out_dict && orig_kb < kb) {
+            DevBuf narrow((size_t)std::max<int64_t>(dv.size, 1) * orig_kb, s);
+            if (dv.size > 0) {
+                hipLaunchKernelGGL(narrow_kernel, dim3((unsigned)small_grid(dv.size)), dim3(256), 0, s,
+                                   (const int32_t*)dv.data, orig_kb, dv.size, narrow.get());
+                EG_LAUNCH_CHECK();
+            }
+            EG_HIP(hipStreamSynchronize(s));
+            out_dict->data = narrow.release();
+            out_dict->size = dv.size;
+            out_dict->type = column.type;
+            out_dict->reserved = 0;
+            out_dict->offset = 0;
+        } else if (out_dict) {
+            out_dict->data = dv.data;
+            dict_owner.p = nullptr;
+            out_dict->size = dv.size;
+            out_dict->type = column.type;
+            out_dict->reserved = 0;
+            out_dict->offset = 0;
+        }
+    });
+    if (t) eg_table_free(t);
+    return rc;
+}

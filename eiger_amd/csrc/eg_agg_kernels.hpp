@@ -85,7 +85,6 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
                                          int cr_log2, int rep_log2, int probe_limit, int peel_min)
 {
     constexpr int NV = VL::NV > 0 ? VL::NV : 1;
-    using KT = typename LdsKey<KB>::T;
     extern __shared__ __align__(16) unsigned long long smem[];
     const bool use_lds = cr_log2 >= 0;
     // Rows reach the HBM table as spills of cold keys (hot keys take LDS

@@ -444,6 +444,33 @@ def compute_partition_offsets(partitioned_key_col, begin_bit, end_bit, stream=No
     return _own(a)
 
 
+# ---------------------------------------------------------------- dictionary encoding
+def dictionary_encode_with_dict(col, dict_col, encode_type=type_id.INT32, stream=None) -> column:
+    """gamax::compress::dictionary_encode(column, dict, encode_type)
+    (compress/dictionary_encode.hpp:26-30, dictionary_encode_dict_ready.cu:15-80):
+    the position of each key in dict_col, INT32_MIN where absent."""
+    a = ColumnView()
+    _check(lib().eg_dictionary_encode_with_dict(col._to_c(), dict_col._to_c(), int(encode_type), _stream(stream),
+                                                C.byref(a)))
+    return _own(a)
+
+
+def dictionary_encode(col, encode_type=type_id.INT32, dictionary_size_hint=0, with_order=False,
+                      return_dictionary=False, stream=None):
+    """gamax::compress::dictionary_encode / dictionary_encode_with_order
+    (compress/dictionary_encode.hpp:12-24): (encoded column, dictionary size),
+    dense ids in [0, size); with_order numbers keys in ascending order.  With
+    return_dictionary the distinct keys (id i at row i) come third."""
+    a, d = ColumnView(), ColumnView()
+    size = C.c_int64()
+    _check(lib().eg_dictionary_encode(col._to_c(), int(encode_type), int(dictionary_size_hint), int(with_order),
+                                      _stream(stream), C.byref(a), C.byref(size),
+                                      C.byref(d) if return_dictionary else None))
+    if return_dictionary:
+        return _own(a), size.value, _own(d)
+    return _own(a), size.value
+
+
 # ---------------------------------------------------------------- statistics / dispatch
 def compute_ndv_approx(col, stream=None) -> int:
     """gamax::stats::compute_ndv_approx (stats/stats_compute.hpp:72)."""

@@ -470,6 +470,41 @@ inline std::unique_ptr<column> compute_partition_offsets(column_view partitioned
 }
 }  // namespace partition
 
+namespace compress {
+// compress/dictionary_encode.hpp:12-37 (integer key columns).  The
+// self-built forms return (encoded, dictionary size); load_factor is
+// accepted for signature parity and not used (the table is sized here).
+inline std::pair<std::unique_ptr<column>, size_type> dictionary_encode(
+    column_view col, data_type encode_type = data_type{type_id::INT32}, size_type dictionary_size_hint = 0,
+    double /*load_factor*/ = 0.25, hipStream_t stream = nullptr)
+{
+    eg_column_view e{};
+    int64_t n = 0;
+    detail::check(eg_dictionary_encode(col.abi(), (int32_t)encode_type.id(), dictionary_size_hint, 0, stream, &e, &n,
+                                       nullptr));
+    return {detail::own(e), n};
+}
+inline std::pair<std::unique_ptr<column>, size_type> dictionary_encode_with_order(
+    column_view col, data_type encode_type = data_type{type_id::INT32}, size_type dictionary_size_hint = 0,
+    double /*load_factor*/ = 0.25, hipStream_t stream = nullptr)
+{
+    eg_column_view e{};
+    int64_t n = 0;
+    detail::check(eg_dictionary_encode(col.abi(), (int32_t)encode_type.id(), dictionary_size_hint, 1, stream, &e, &n,
+                                       nullptr));
+    return {detail::own(e), n};
+}
+// dictionary_encode_dict_ready.cu:15-80: positions in `dict`, INT32_MIN if absent.
+inline std::unique_ptr<column> dictionary_encode(column_view col, column_view dict,
+                                                 data_type encode_type = data_type{type_id::INT32},
+                                                 hipStream_t stream = nullptr)
+{
+    eg_column_view e{};
+    detail::check(eg_dictionary_encode_with_dict(col.abi(), dict.abi(), (int32_t)encode_type.id(), stream, &e));
+    return detail::own(e);
+}
+}  // namespace compress
+
 namespace stats {
 // stats/stats_compute.hpp:72
 inline size_type compute_ndv_approx(column_view col, hipStream_t stream = nullptr)

@@ -242,6 +242,25 @@ int eg_compute_partition_offsets(eg_column_view partitioned_key, int32_t begin_b
                                  int32_t end_bit, void* stream,
                                  eg_column_view* out_offsets);
 
+/* ---- dictionary encoding (gamax::compress, compress/dictionary_encode.hpp:12-37) --
+ * Integer (and BOOL8) key columns only.  The encoded column is a new device
+ * allocation owned by the caller (free with eg_free). */
+/* dictionary_encode(column, dict, encode_type) (dictionary_encode_dict_ready.cu:15-80):
+ * each row gets the position of its key in `dict` (same type as column), or
+ * INT32_MIN when absent; a key present more than once in dict gets its first
+ * position.  encode_type must be EG_INT32. */
+int eg_dictionary_encode_with_dict(eg_column_view column, eg_column_view dict, int32_t encode_type,
+                                   void* stream, eg_column_view* out);
+/* dictionary_encode / dictionary_encode_with_order (dictionary_encode.hpp:12-24):
+ * each row gets a dense id in [0, *out_dict_size); with_order != 0 numbers the
+ * distinct keys in ascending key order.  encode_type: any integer type wide
+ * enough for the number of distinct keys (else EG_ERR_INVALID).
+ * dictionary_size_hint (0 = none) sizes the distinct-key pass.  out_dict
+ * (optional, NULL = not returned) receives the distinct keys, id i at row i. */
+int eg_dictionary_encode(eg_column_view column, int32_t encode_type, int64_t dictionary_size_hint,
+                         int32_t with_order, void* stream, eg_column_view* out, int64_t* out_dict_size,
+                         eg_column_view* out_dict);
+
 /* ---- statistics and dispatch ----------------------------------------------- */
 /* gamax::stats::compute_ndv_approx (stats/stats_compute.hpp:72, stats_compute.cu:103-196):
  * HyperLogLog over the whole column, 2^14 registers. */

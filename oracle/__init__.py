@@ -81,6 +81,10 @@ def lib():
         L.orc_fmix64.restype = u64
         L.orc_fmix64.argtypes = [u64]
         L.orc_partition_ids.argtypes = [vp, i32, i64, u32, vp]
+        L.orc_dict_encode.argtypes = [vp, vp, i32, i64, i64, i64, vp]
+        L.orc_dict_encode.restype = C.c_int
+        L.orc_dict_encode_ordered.argtypes = [vp, i32, i64, vp]
+        L.orc_dict_encode_ordered.restype = i64
         L.orc_agg_output_type.restype = i32
         L.orc_agg_output_type.argtypes = [i32, i32]
         L.orc_groupby.restype = C.c_int
@@ -288,6 +292,33 @@ def rule_select(key_bytes: int, ndv: int) -> int:
 def count_distinct(keys) -> int:
     keys = np.ascontiguousarray(keys)
     return int(lib().orc_count_distinct(_ptr(keys), type_id(keys.dtype), len(keys)))
+
+
+INT32_MIN = -(1 << 31)
+
+
+def dict_encode(keys, dict_keys, miss=INT32_MIN):
+    """Position of each key in dict_keys (first one if repeated), `miss` when absent."""
+    keys = np.ascontiguousarray(keys)
+    dict_keys = np.ascontiguousarray(dict_keys)
+    if keys.dtype != dict_keys.dtype:
+        raise OracleError("dictionary and column types differ")
+    out = np.empty(len(keys), dtype=np.int64)
+    rc = lib().orc_dict_encode(_ptr(keys), _ptr(dict_keys), type_id(keys.dtype), len(keys), len(dict_keys), miss,
+                               _ptr(out))
+    if rc != 0:
+        raise OracleError(lib().orc_last_error().decode())
+    return out
+
+
+def dict_encode_ordered(keys):
+    """(rank of each key among the distinct keys in ascending order, number of distinct keys)."""
+    keys = np.ascontiguousarray(keys)
+    out = np.empty(len(keys), dtype=np.int64)
+    d = int(lib().orc_dict_encode_ordered(_ptr(keys), type_id(keys.dtype), len(keys), _ptr(out)))
+    if d < 0:
+        raise OracleError("out of memory")
+    return out, d
 
 
 # ---------------------------------------------------------------- learned dispatch

@@ -969,3 +969,52 @@ int64_t orc_count_distinct(const void* keys, int32_t type, int64_t n)
     free(t);
     return d;
 }
+
+/* compress/dictionary_encode_dict_ready.cu:15-80 (cuco static_map insert of
+ * (dict[i], i), then find): out[r] = the position of keys[r] in dict, `miss`
+ * (the reference's sentinel<int32_t>() = INT32_MIN) when absent.  A key held
+ * more than once in dict maps to its first position (the reference's map keeps
+ * whichever insert wins; dictionaries there hold unique keys).  Stable sort of
+ * (key, position) + lower-bound search. */
+int orc_dict_encode(const void* keys, const void* dict, int32_t type, int64_t n, int64_t m, int64_t miss,
+                    int64_t* out)
+{
+    kr_t* a = (kr_t*)malloc((size_t)(m > 0 ? m : 1) * sizeof(kr_t));
+    kr_t* t = (kr_t*)malloc((size_t)(m > 0 ? m : 1) * sizeof(kr_t));
+    if (!a || !t) { free(a); free(t); return fail("out of memory"); }
+    for (int64_t i = 0; i < m; ++i) { a[i].k = key_ord(dict, type, i); a[i].r = i; }
+    merge_sort_kr(a, t, m);
+    for (int64_t r = 0; r < n; ++r) {
+        const uint64_t k = key_ord(keys, type, r);
+        int64_t lo = 0, hi = m;
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if (a[mid].k < k) lo = mid + 1; else hi = mid;
+        }
+        out[r] = (lo < m && a[lo].k == k) ? a[lo].r : miss;
+    }
+    free(a);
+    free(t);
+    return 0;
+}
+
+/* compress/dictionary_encode.cu (dictionary_encode_with_order): out[r] = the
+ * rank of keys[r] among the distinct keys in ascending order; returns the
+ * number of distinct keys, -1 on allocation failure. */
+int64_t orc_dict_encode_ordered(const void* keys, int32_t type, int64_t n, int64_t* out)
+{
+    if (n <= 0) return 0;
+    kr_t* a = (kr_t*)malloc((size_t)n * sizeof(kr_t));
+    kr_t* t = (kr_t*)malloc((size_t)n * sizeof(kr_t));
+    if (!a || !t) { free(a); free(t); return -1; }
+    for (int64_t i = 0; i < n; ++i) { a[i].k = key_ord(keys, type, i); a[i].r = i; }
+    merge_sort_kr(a, t, n);
+    int64_t d = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (i > 0 && a[i].k != a[i - 1].k) ++d;
+        out[a[i].r] = d;
+    }
+    free(a);
+    free(t);
+    return d + 1;
+}

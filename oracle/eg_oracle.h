@@ -117,6 +117,11 @@ int orc_hash_partition(const void* keys, int32_t key_type, int64_t n,
 int32_t orc_rule_select(int32_t key_bytes, int64_t ndv);
 int64_t orc_count_distinct(const void* keys, int32_t type, int64_t n);
 
+/* ---- dictionary encoding ------------------------------------------------- */
+int orc_dict_encode(const void* keys, const void* dict, int32_t type, int64_t n, int64_t m, int64_t miss,
+                    int64_t* out);
+int64_t orc_dict_encode_ordered(const void* keys, int32_t type, int64_t n, int64_t* out);
+
 /* ---- learned dispatch (eg_oracle_mlp.c): fp64 restatement ---------------- */
 #define ORC_MLP_PARAMS 5508
 int orc_mlp_forward(const double* params, const float* x, int64_t n, double* y);

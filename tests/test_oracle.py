@@ -455,3 +455,44 @@ def test_sample_features_match_numpy(kt, rows, d):
            np.log2(float(s.max() - s.min() + 1) + 1) / 64, np.log2(rows + 1) / 32, 5 / 16, 12 / 32,
            (np.diff(s) >= 0).sum() / (ns - 1), ns / 4096]
     np.testing.assert_allclose(f, np.array(exp, np.float32), rtol=2e-6, atol=1e-7)
+
+
+# ---------------------------------------------------------------- dictionary encoding
+@pytest.mark.parametrize("d", [1, 16, 4096])
+def test_dict_encode_reference_case_is_identity(d):
+    """dictionary_encode_dict_ready_test.cu:59-92: dict = 0..D-1, keys =
+    (0..n-1) mod D shuffled, so each key's position is the key itself."""
+    rng = np.random.default_rng(d)
+    keys = (np.arange(4096, dtype=np.int32) % d)
+    rng.shuffle(keys)
+    np.testing.assert_array_equal(O.dict_encode(keys, np.arange(d, dtype=np.int32)), keys.astype(np.int64))
+
+
+@pytest.mark.parametrize("dt", [np.int8, np.uint8, np.int16, np.int32, np.uint32, np.int64, np.uint64])
+def test_dict_encode_matches_bruteforce(dt):
+    """First position in the dictionary, INT32_MIN when absent; extremes of the
+    type (the reference's empty-key sentinel among them) are ordinary keys."""
+    rng = np.random.default_rng(7)
+    info = np.iinfo(dt)
+    pool = np.array([info.min, info.max, 0, 1, 2, 3, 5, 8, 13], dtype=dt)
+    dict_keys = rng.choice(pool, 7)  # repeats allowed
+    keys = rng.choice(pool, 200)
+    dl = dict_keys.tolist()
+    exp = [dl.index(k) if k in dl else O.INT32_MIN for k in keys.tolist()]
+    np.testing.assert_array_equal(O.dict_encode(keys, dict_keys), exp)
+    assert (O.dict_encode(keys, dict_keys[:0]) == O.INT32_MIN).all()
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.int64, np.uint16])
+def test_dict_encode_ordered_matches_numpy_unique(dt):
+    rng = np.random.default_rng(11)
+    info = np.iinfo(dt)
+    keys = rng.integers(info.min, info.max, 5000, dtype=dt, endpoint=True) % dt(97) if dt == np.uint16 else \
+        rng.choice(rng.integers(info.min, info.max, 300, dtype=dt, endpoint=True), 5000)
+    uniq, inv = np.unique(keys, return_inverse=True)
+    ids, d = O.dict_encode_ordered(keys)
+    assert d == len(uniq)
+    np.testing.assert_array_equal(ids, inv.reshape(-1))
+    # the ordered ids are the positions in the sorted distinct keys
+    np.testing.assert_array_equal(O.dict_encode(keys, uniq), ids)
+    assert O.dict_encode_ordered(keys[:0])[1] == 0
