@@ -34,9 +34,11 @@ constexpr int kDictBlock = 1024;
 constexpr int kLdsSlotsMax = 8192;  // 32 KB of positions + up to 64 KB of keys
 constexpr int64_t kMaxDict = (int64_t)1 << 30;
 
+// First slot of the key's bucket: buckets are aligned groups of 4 slots, and
+// probing runs linearly from there, so one 16-byte read covers a bucket.
 __device__ __forceinline__ uint32_t dict_home(uint64_t raw, int cap_log2)
 {
-    return (uint32_t)(dev::fmix64(raw) >> (64 - cap_log2));
+    return (uint32_t)(dev::fmix64(raw) >> (64 - (cap_log2 - 2))) << 2;
 }
 
 __global__ __launch_bounds__(256) void dict_build_kernel(const void* dict, int kb, int64_t m, uint32_t* slots,
@@ -49,7 +51,9 @@ __global__ __launch_bounds__(256) void dict_build_kernel(const void* dict, int k
         const uint32_t v = (uint32_t)i + 1u;
         uint32_t h = dict_home(k, cap_log2);
         // at most m of the 2^cap_log2 >= 2m slots are ever taken, so an empty
-        // slot is met within the bound
+        // slot is met within the bound.  Slots are taken in probe order and
+        // never freed, so after the build the occupied slots of every probe
+        // sequence form a prefix: a lookup stops at the first empty slot.
         for (uint32_t p = 0; p <= mask; ++p) {
             const uint32_t old = atomicCAS(&slots[h], 0u, v);
             if (old == 0u) break;
@@ -72,8 +76,10 @@ __global__ __launch_bounds__(256) void dict_slot_keys_kernel(const void* dict, i
     }
 }
 
-template <int KB>
-using SlotKey = typename std::conditional<(KB > 4), uint64_t, uint32_t>::type;
+// Keys of the LDS copy: 4 bytes for keys of up to 4 bytes, else 8.  The HBM
+// table keeps 8-byte keys.
+template <int KB, bool LDS>
+using SlotKey = typename std::conditional<(KB > 4 || !LDS), uint64_t, uint32_t>::type;
 
 template <int KB>
 __device__ __forceinline__ uint64_t load_key(const void* p, int64_t i)
@@ -93,18 +99,57 @@ __device__ __forceinline__ void store_id(void* p, int64_t i, uint64_t bits)
     else ((uint64_t*)p)[i] = bits;
 }
 
-// Dictionary position of k, or `miss`.
+// One bucket (4 slots from h, h a multiple of 4): 16-byte reads.
 template <typename KT>
-__device__ __forceinline__ uint64_t probe(const uint32_t* sidx, const KT* skey, uint64_t k, int cap_log2,
-                                          uint64_t miss)
+struct DictBucket {
+    uint32_t v[4];
+    KT k[4];
+};
+
+template <typename KT>
+__device__ __forceinline__ DictBucket<KT> read_bucket(const uint32_t* sidx, const KT* skey, uint32_t h)
+{
+    DictBucket<KT> b;
+    const uint4 v = *(const uint4*)(sidx + h);
+    b.v[0] = v.x, b.v[1] = v.y, b.v[2] = v.z, b.v[3] = v.w;
+    if constexpr (sizeof(KT) == 4) {
+        const uint4 k = *(const uint4*)(skey + h);
+        b.k[0] = k.x, b.k[1] = k.y, b.k[2] = k.z, b.k[3] = k.w;
+    } else {
+        const ulonglong2 k0 = *(const ulonglong2*)(skey + h), k1 = *(const ulonglong2*)(skey + h + 2);
+        b.k[0] = k0.x, b.k[1] = k0.y, b.k[2] = k1.x, b.k[3] = k1.y;
+    }
+    return b;
+}
+
+// true when the bucket decides the lookup (res = position or miss).
+template <typename KT>
+__device__ __forceinline__ bool match_bucket(const DictBucket<KT>& b, uint64_t k, uint64_t miss, uint64_t& res)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (b.v[j] == 0u) {
+            res = miss;
+            return true;
+        }
+        if ((uint64_t)b.k[j] == k) {
+            res = (uint64_t)(b.v[j] - 1u);
+            return true;
+        }
+    }
+    return false;
+}
+
+// Lookup continued from bucket h (the key's earlier buckets were full).
+template <typename KT>
+__device__ __noinline__ uint64_t probe_from(const uint32_t* sidx, const KT* skey, uint64_t k, uint32_t h,
+                                            int cap_log2, uint64_t miss)
 {
     const uint32_t mask = (1u << cap_log2) - 1u;
-    uint32_t h = dict_home(k, cap_log2);
-    for (uint32_t p = 0; p <= mask; ++p) {
-        const uint32_t v = sidx[h];
-        if (v == 0u) break;
-        if ((uint64_t)skey[h] == k) return (uint64_t)(v - 1u);
-        h = (h + 1u) & mask;
+    for (uint32_t p = 0; p <= mask; p += 4) {
+        uint64_t r;
+        if (match_bucket(read_bucket(sidx, skey, h), k, miss, r)) return r;
+        h = (h + 4u) & mask;
     }
     return miss;
 }
@@ -115,10 +160,10 @@ __global__ __launch_bounds__(kDictBlock) void dict_find_kernel(const void* __res
                                                                const uint64_t* __restrict__ skeys, int cap_log2,
                                                                uint64_t miss, void* __restrict__ out)
 {
-    using KT = SlotKey<KB>;
+    using KT = SlotKey<KB, LDS>;
     extern __shared__ __align__(16) unsigned char dict_smem[];
     const uint32_t* sidx = slots;
-    const KT* skey = nullptr;
+    const KT* skey = (const KT*)skeys;
     if constexpr (LDS) {
         const int cap = 1 << cap_log2;
         uint32_t* li = (uint32_t*)dict_smem;
@@ -130,25 +175,32 @@ __global__ __launch_bounds__(kDictBlock) void dict_find_kernel(const void* __res
         __syncthreads();
         sidx = li;
         skey = lk;
-    } else {
-        static_assert(sizeof(KT) <= 8, "slot keys");
-        if constexpr (sizeof(KT) == 8) skey = (const KT*)skeys;
     }
+    const uint32_t mask = (1u << cap_log2) - 1u;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if constexpr (!LDS && sizeof(KT) != 8) {
-        // HBM slot keys are 64-bit words
-        for (; i < n; i += stride) store_id<OB>(out, i, probe(sidx, skeys, load_key<KB>(keys, i), cap_log2, miss));
-        return;
-    } else {
-        for (; i + 3 * stride < n; i += 4 * stride) {
-            uint64_t k[4];
+    // four rows per step: their first buckets are read together
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        uint64_t k[4];
+        uint32_t h[4];
+        DictBucket<KT> b[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) k[u] = load_key<KB>(keys, i + u * stride);
+        for (int u = 0; u < 4; ++u) k[u] = load_key<KB>(keys, i + u * stride);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) store_id<OB>(out, i + u * stride, probe(sidx, skey, k[u], cap_log2, miss));
+        for (int u = 0; u < 4; ++u) {
+            h[u] = dict_home(k[u], cap_log2);
+            b[u] = read_bucket(sidx, skey, h[u]);
         }
-        for (; i < n; i += stride) store_id<OB>(out, i, probe(sidx, skey, load_key<KB>(keys, i), cap_log2, miss));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            uint64_t r;
+            if (!match_bucket(b[u], k[u], miss, r)) r = probe_from(sidx, skey, k[u], (h[u] + 4u) & mask, cap_log2, miss);
+            store_id<OB>(out, i + u * stride, r);
+        }
+    }
+    for (; i < n; i += stride) {
+        const uint64_t k = load_key<KB>(keys, i);
+        store_id<OB>(out, i, probe_from(sidx, skey, k, dict_home(k, cap_log2), cap_log2, miss));
     }
 }
 
@@ -159,7 +211,7 @@ void launch_find(const void* keys, int64_t n, const uint32_t* slots, const uint6
     const void* kern = (const void*)dict_find_kernel<KB, OB, LDS>;
     size_t lds = 0;
     if (LDS) {
-        lds = ((size_t)4 + sizeof(SlotKey<KB>)) << cap_log2;
+        lds = ((size_t)4 + sizeof(SlotKey<KB, true>)) << cap_log2;
         if (lds > enable_dynamic_lds(kern)) raise(EG_ERR_DEVICE, "dictionary table exceeds LDS");
     }
     int per_cu = 0;
