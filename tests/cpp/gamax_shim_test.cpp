@@ -222,6 +222,33 @@ static void test_partition_and_sort()
     pass("partition_and_sort");
 }
 
+// compress/dictionary_encode_dict_ready_test.cu:59-92 through the C++ mirror:
+// dict = 0..D-1, keys (0..n-1) mod D shuffled, so each key's position is the
+// key; an absent key gives INT32_MIN; the self-built ordered ids of the same
+// keys are the keys again (their distinct values are exactly 0..D-1).
+static void test_dictionary_encode()
+{
+    const int32_t n = 4096, d = 16;
+    std::vector<int32_t> hk((size_t)n), hd((size_t)d);
+    for (int32_t i = 0; i < n; ++i) hk[(size_t)i] = i % d;
+    std::shuffle(hk.begin(), hk.end(), std::mt19937(3));
+    for (int32_t i = 0; i < d; ++i) hd[(size_t)i] = i;
+    hk[5] = 1000;  // absent from the dictionary
+    auto kc = make_column(hk, data_type(type_id::INT32));
+    auto dc = make_column(hd, data_type(type_id::INT32));
+    auto e = copy_column_to_host<int32_t>(compress::dictionary_encode(kc->view(), dc->view())->view());
+    for (int32_t i = 0; i < n; ++i) CHECK(e[(size_t)i] == (i == 5 ? INT32_MIN : hk[(size_t)i]));
+    hk[5] = 0;
+    auto kc2 = make_column(hk, data_type(type_id::INT32));
+    auto [ids, size] = compress::dictionary_encode_with_order(kc2->view(), data_type(type_id::INT64));
+    CHECK(size == d);
+    auto hid = copy_column_to_host<int64_t>(ids->view());
+    for (int32_t i = 0; i < n; ++i) CHECK(hid[(size_t)i] == hk[(size_t)i]);
+    auto [ids2, size2] = compress::dictionary_encode(kc2->view());
+    CHECK(size2 == d && ids2->size() == n);
+    pass("dictionary_encode");
+}
+
 // phys/phys_test.cu:455-476,520-548: the engine-facing wrappers.  Empty
 // inputs: group_by gives 0 rows with schema [key, SUM output]; the scalar
 // aggregate gives one row of zeros.  Keys {1,1,2,2,3}, values {10..50}:
@@ -383,6 +410,7 @@ int main()
     test_partition_and_sort();
     test_phys_wrappers();
     test_pullops_group_by();
+    test_dictionary_encode();
     test_errors();
     std::printf("ALL PASSED\n");
     return g_failed;
