@@ -26,6 +26,7 @@
 
 #include "eg_device.hpp"
 #include "eg_internal.hpp"
+#include "eg_radix.hpp"
 
 namespace eg {
 namespace {
@@ -204,6 +205,77 @@ __global__ __launch_bounds__(kDictBlock) void dict_find_kernel(const void* __res
     }
 }
 
+// Direct-indexed table for dictionaries whose key span (max - min + 1) is
+// at most kDirectMax: slot d = key - min holds ~position (0 = absent;
+// atomicMax keeps the first position of a repeated key), and a row costs one
+// 4-byte LDS read.  key - min is taken modulo 2^(8 KB), so keys outside
+// [min, max] land above the span without a signed compare.
+constexpr int64_t kDirectMax = 16384;
+
+__global__ __launch_bounds__(256) void dict_direct_build_kernel(const void* dict, int kb, int64_t m, uint64_t raw_min,
+                                                                uint64_t wmask, uint32_t* table)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
+        atomicMax(&table[(dev::load_raw(dict, kb, i) - raw_min) & wmask], ~(uint32_t)i);
+}
+
+template <int KB, int OB>
+__global__ __launch_bounds__(kDictBlock) void dict_direct_kernel(const void* __restrict__ keys, int64_t n,
+                                                                 const uint32_t* __restrict__ table, uint64_t span,
+                                                                 uint64_t raw_min, uint64_t miss, void* __restrict__ out)
+{
+    extern __shared__ __align__(16) unsigned char dict_smem[];
+    uint32_t* dt = (uint32_t*)dict_smem;
+    for (int64_t s = threadIdx.x; s <= (int64_t)span; s += blockDim.x) dt[s] = table[s];
+    __syncthreads();
+    constexpr uint64_t wmask = KB == 8 ? ~0ull : ((1ull << (8 * KB)) - 1ull);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    auto find = [&](uint64_t k) -> uint64_t {
+        const uint64_t d = (k - raw_min) & wmask;
+        const uint32_t v = d <= span ? dt[d] : 0u;
+        return v ? (uint64_t)(uint32_t)~v : miss;
+    };
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        uint64_t k[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) k[u] = load_key<KB>(keys, i + u * stride);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) store_id<OB>(out, i + u * stride, find(k[u]));
+    }
+    for (; i < n; i += stride) store_id<OB>(out, i, find(load_key<KB>(keys, i)));
+}
+
+template <int KB, int OB>
+void launch_direct(const void* keys, int64_t n, const uint32_t* table, uint64_t span, uint64_t raw_min, uint64_t miss,
+                   void* out, hipStream_t s)
+{
+    const void* kern = (const void*)dict_direct_kernel<KB, OB>;
+    const size_t lds = (size_t)(span + 1) * 4;
+    if (lds > enable_dynamic_lds(kern)) raise(EG_ERR_DEVICE, "dictionary table exceeds LDS");
+    int per_cu = 0;
+    EG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDictBlock, lds));
+    per_cu = std::max(per_cu, 1);
+    const int64_t grid =
+        std::max<int64_t>(1, std::min<int64_t>((n + kDictBlock - 1) / kDictBlock, (int64_t)device_info().num_cus * per_cu));
+    hipLaunchKernelGGL((dict_direct_kernel<KB, OB>), dim3((unsigned)grid), dim3(kDictBlock), lds, s, keys, n, table,
+                       span, raw_min, miss, out);
+    EG_LAUNCH_CHECK();
+}
+
+template <int KB>
+void launch_direct_ob(int ob, const void* keys, int64_t n, const uint32_t* table, uint64_t span, uint64_t raw_min,
+                      uint64_t miss, void* out, hipStream_t s)
+{
+    switch (ob) {
+    case 1: launch_direct<KB, 1>(keys, n, table, span, raw_min, miss, out, s); break;
+    case 2: launch_direct<KB, 2>(keys, n, table, span, raw_min, miss, out, s); break;
+    case 4: launch_direct<KB, 4>(keys, n, table, span, raw_min, miss, out, s); break;
+    default: launch_direct<KB, 8>(keys, n, table, span, raw_min, miss, out, s); break;
+    }
+}
+
 template <int KB, int OB, bool LDS>
 void launch_find(const void* keys, int64_t n, const uint32_t* slots, const uint64_t* skeys, int cap_log2,
                  uint64_t miss, void* out, hipStream_t s)
@@ -289,14 +361,38 @@ void check_key_type(int32_t t)
 
 // Positions of column's keys in dict (n rows of `out_type`, `miss` where a key
 // is absent).  Both columns have the same integer type.
-DevBuf dict_encode(const void* keys, int64_t n, const void* dict, int64_t m, int kb, int32_t out_type, uint64_t miss,
-                   hipStream_t s)
+DevBuf dict_encode(const void* keys, int64_t n, const void* dict, int64_t m, int32_t type, int32_t out_type,
+                   uint64_t miss, hipStream_t s)
 {
+    const int kb = type_size(type);
     const int ob = type_size(out_type);
     DevBuf out((size_t)std::max<int64_t>(n, 1) * ob, s);
     if (n == 0) return out;
     if (m > kMaxDict) raise(EG_ERR_UNSUPPORTED, "dictionary larger than 2^30 entries");
-    // an empty dictionary gives an empty 64-slot table: every key is absent
+    if (m > 0) {
+        uint64_t tmin = 0, tmax = 0;
+        key_range(dict, type, kb, m, s, &tmin, &tmax);  // order-preserving form
+        const uint64_t span = tmax - tmin;
+        if (span < (uint64_t)kDirectMax) {
+            const uint64_t raw_min = twiddle_bits(tmin, type, kb);  // the twiddle of an integer is its own inverse
+            const uint64_t wmask = kb == 8 ? ~0ull : ((1ull << (8 * kb)) - 1ull);
+            DevBuf table((size_t)(span + 1) * 4, s);
+            fill_zero(table.get(), (size_t)(span + 1) * 4, s);
+            hipLaunchKernelGGL(dict_direct_build_kernel, dim3((unsigned)small_grid(m)), dim3(256), 0, s, dict, kb, m,
+                               raw_min, wmask, table.get<uint32_t>());
+            EG_LAUNCH_CHECK();
+            const uint32_t* t = table.get<uint32_t>();
+            switch (kb) {
+            case 1: launch_direct_ob<1>(ob, keys, n, t, span, raw_min, miss, out.get(), s); break;
+            case 2: launch_direct_ob<2>(ob, keys, n, t, span, raw_min, miss, out.get(), s); break;
+            case 4: launch_direct_ob<4>(ob, keys, n, t, span, raw_min, miss, out.get(), s); break;
+            default: launch_direct_ob<8>(ob, keys, n, t, span, raw_min, miss, out.get(), s); break;
+            }
+            EG_HIP(hipStreamSynchronize(s));  // the table is freed on return
+            return out;
+        }
+    }
+    // hashed table; an empty dictionary gives an empty 64-slot table (every key absent)
     int cap_log2 = 6;
     while (((int64_t)1 << cap_log2) < 2 * m) ++cap_log2;
     const int64_t cap = (int64_t)1 << cap_log2;
@@ -336,7 +432,7 @@ int eg_dictionary_encode_with_dict(eg_column_view column, eg_column_view dict, i
         const void* keys = view_ptr(column, kb);
         const void* d = view_ptr(dict, kb);
         hipStream_t s = (hipStream_t)stream;
-        DevBuf enc = dict_encode(keys, column.size, d, dict.size, kb, EG_INT32, (uint64_t)(uint32_t)INT32_MIN, s);
+        DevBuf enc = dict_encode(keys, column.size, d, dict.size, column.type, EG_INT32, (uint64_t)(uint32_t)INT32_MIN, s);
         out->data = enc.release();
         out->size = column.size;
         out->type = EG_INT32;
@@ -391,7 +487,7 @@ int eg_dictionary_encode(eg_column_view column, int32_t encode_type, int64_t dic
             const int vb = 8 * ob - (is_signed_int(encode_type) ? 1 : 0);
             if (dv.size > ((int64_t)1 << vb)) raise(EG_ERR_INVALID, "dictionary does not fit the encoded type");
         }
-        DevBuf enc = dict_encode(keys, column.size, dv.data, dv.size, kb, encode_type, 0ull, s);
+        DevBuf enc = dict_encode(keys, column.size, dv.data, dv.size, src.type, encode_type, 0ull, s);
         out->data = enc.release();
         out->size = column.size;
         out->type = encode_type;
