@@ -274,6 +274,7 @@ def main():
         "batched": batched,
         "sweep": sweep,
     }
+    G.jit_wait()  # compiles queued by the batched comparison finish before exit
     print(json.dumps(out), flush=True)
 
 

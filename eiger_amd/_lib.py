@@ -6,6 +6,7 @@ library has not been built; nothing here falls back to a CPU path.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 
@@ -160,7 +161,15 @@ def lib():
         L = C.CDLL(LIB_PATH)
         _proto(L)
         _lib = L
+        atexit.register(_drain_compiles)
     return _lib
+
+
+def _drain_compiles():
+    """Interpreter exit: lets queued run-time compiles finish while the
+    process is still whole, so none is running once native teardown starts."""
+    if _lib is not None:
+        _lib.eg_jit_wait()
 
 
 def header_functions(path: str = HEADER):
