@@ -17,7 +17,7 @@
 // second pass stores each slot's key beside it.  The lookup streams the column
 // once (read key, write id): HBM-bound, 4 + 4 bytes per int32 row.  Tables of
 // up to 8192 slots are copied into each workgroup's LDS; larger ones are probed
-// in HBM (they stay L2-resident up to a few MB).  Unlike the reference's cuco
+// in HBM as packed 16-byte {key, position} slots, one 64-byte line per bucket.  Unlike the reference's cuco
 // map (INT_MIN reserved as the empty key), every key value is valid.
 #include <hip/hip_runtime.h>
 
@@ -74,6 +74,17 @@ __global__ __launch_bounds__(256) void dict_slot_keys_kernel(const void* dict, i
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += stride) {
         const uint32_t v = slots[s];
         skeys[s] = v ? dev::load_raw(dict, kb, (int64_t)v - 1) : 0ull;
+    }
+}
+
+__global__ __launch_bounds__(256) void dict_pack_kernel(const void* dict, int kb, const uint32_t* slots, int64_t cap,
+                                                        uint4* packed)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += stride) {
+        const uint32_t v = slots[s];
+        const uint64_t k = v ? dev::load_raw(dict, kb, (int64_t)v - 1) : 0ull;
+        packed[s] = make_uint4((uint32_t)k, (uint32_t)(k >> 32), v, 0u);
     }
 }
 
@@ -141,42 +152,49 @@ __device__ __forceinline__ bool match_bucket(const DictBucket<KT>& b, uint64_t k
     return false;
 }
 
-// Lookup continued from bucket h (the key's earlier buckets were full).
+// Where a lookup reads its buckets: the LDS copy (positions and keys in two
+// arrays) or the packed HBM table (one 16-byte {key, position} entry per
+// slot, so a bucket is one 64-byte line: one random line per lookup instead
+// of two).
 template <typename KT>
-__device__ __noinline__ uint64_t probe_from(const uint32_t* sidx, const KT* skey, uint64_t k, uint32_t h,
-                                            int cap_log2, uint64_t miss)
+struct SplitTable {
+    const uint32_t* sidx;
+    const KT* skey;
+    __device__ __forceinline__ DictBucket<KT> bucket(uint32_t h) const { return read_bucket(sidx, skey, h); }
+};
+
+struct PackedTable {
+    const uint4* slot;
+    __device__ __forceinline__ DictBucket<uint64_t> bucket(uint32_t h) const
+    {
+        DictBucket<uint64_t> b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint4 t = slot[h + j];
+            b.v[j] = t.z;
+            b.k[j] = (uint64_t)t.x | ((uint64_t)t.y << 32);
+        }
+        return b;
+    }
+};
+
+// Lookup continued from bucket h (the key's earlier buckets were full).
+template <class TAB>
+__device__ __noinline__ uint64_t probe_from(const TAB& t, uint64_t k, uint32_t h, int cap_log2, uint64_t miss)
 {
     const uint32_t mask = (1u << cap_log2) - 1u;
     for (uint32_t p = 0; p <= mask; p += 4) {
         uint64_t r;
-        if (match_bucket(read_bucket(sidx, skey, h), k, miss, r)) return r;
+        if (match_bucket(t.bucket(h), k, miss, r)) return r;
         h = (h + 4u) & mask;
     }
     return miss;
 }
 
-template <int KB, int OB, bool LDS>
-__global__ __launch_bounds__(kDictBlock) void dict_find_kernel(const void* __restrict__ keys, int64_t n,
-                                                               const uint32_t* __restrict__ slots,
-                                                               const uint64_t* __restrict__ skeys, int cap_log2,
-                                                               uint64_t miss, void* __restrict__ out)
+template <int KB, int OB, class TAB>
+__device__ __forceinline__ void find_rows(const TAB& t, const void* __restrict__ keys, int64_t n, int cap_log2,
+                                          uint64_t miss, void* __restrict__ out)
 {
-    using KT = SlotKey<KB, LDS>;
-    extern __shared__ __align__(16) unsigned char dict_smem[];
-    const uint32_t* sidx = slots;
-    const KT* skey = (const KT*)skeys;
-    if constexpr (LDS) {
-        const int cap = 1 << cap_log2;
-        uint32_t* li = (uint32_t*)dict_smem;
-        KT* lk = (KT*)(dict_smem + (size_t)cap * 4);
-        for (int s = threadIdx.x; s < cap; s += blockDim.x) {
-            li[s] = slots[s];
-            lk[s] = (KT)skeys[s];
-        }
-        __syncthreads();
-        sidx = li;
-        skey = lk;
-    }
     const uint32_t mask = (1u << cap_log2) - 1u;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -184,24 +202,50 @@ __global__ __launch_bounds__(kDictBlock) void dict_find_kernel(const void* __res
     for (; i + 3 * stride < n; i += 4 * stride) {
         uint64_t k[4];
         uint32_t h[4];
-        DictBucket<KT> b[4];
+        decltype(t.bucket(0u)) b[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) k[u] = load_key<KB>(keys, i + u * stride);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             h[u] = dict_home(k[u], cap_log2);
-            b[u] = read_bucket(sidx, skey, h[u]);
+            b[u] = t.bucket(h[u]);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             uint64_t r;
-            if (!match_bucket(b[u], k[u], miss, r)) r = probe_from(sidx, skey, k[u], (h[u] + 4u) & mask, cap_log2, miss);
+            if (!match_bucket(b[u], k[u], miss, r)) r = probe_from(t, k[u], (h[u] + 4u) & mask, cap_log2, miss);
             store_id<OB>(out, i + u * stride, r);
         }
     }
     for (; i < n; i += stride) {
         const uint64_t k = load_key<KB>(keys, i);
-        store_id<OB>(out, i, probe_from(sidx, skey, k, dict_home(k, cap_log2), cap_log2, miss));
+        store_id<OB>(out, i, probe_from(t, k, dict_home(k, cap_log2), cap_log2, miss));
+    }
+}
+
+// LDS: `slots` positions and `aux` 64-bit keys, copied into LDS per
+// workgroup; HBM: `aux` is the packed table.
+template <int KB, int OB, bool LDS>
+__global__ __launch_bounds__(kDictBlock) void dict_find_kernel(const void* __restrict__ keys, int64_t n,
+                                                               const uint32_t* __restrict__ slots,
+                                                               const void* __restrict__ aux, int cap_log2,
+                                                               uint64_t miss, void* __restrict__ out)
+{
+    if constexpr (LDS) {
+        using KT = SlotKey<KB, true>;
+        extern __shared__ __align__(16) unsigned char dict_smem[];
+        const int cap = 1 << cap_log2;
+        uint32_t* li = (uint32_t*)dict_smem;
+        KT* lk = (KT*)(dict_smem + (size_t)cap * 4);
+        const uint64_t* skeys = (const uint64_t*)aux;
+        for (int s = threadIdx.x; s < cap; s += blockDim.x) {
+            li[s] = slots[s];
+            lk[s] = (KT)skeys[s];
+        }
+        __syncthreads();
+        find_rows<KB, OB>(SplitTable<KT>{li, lk}, keys, n, cap_log2, miss, out);
+    } else {
+        find_rows<KB, OB>(PackedTable{(const uint4*)aux}, keys, n, cap_log2, miss, out);
     }
 }
 
@@ -277,7 +321,7 @@ void launch_direct_ob(int ob, const void* keys, int64_t n, const uint32_t* table
 }
 
 template <int KB, int OB, bool LDS>
-void launch_find(const void* keys, int64_t n, const uint32_t* slots, const uint64_t* skeys, int cap_log2,
+void launch_find(const void* keys, int64_t n, const uint32_t* slots, const void* skeys, int cap_log2,
                  uint64_t miss, void* out, hipStream_t s)
 {
     const void* kern = (const void*)dict_find_kernel<KB, OB, LDS>;
@@ -297,7 +341,7 @@ void launch_find(const void* keys, int64_t n, const uint32_t* slots, const uint6
 }
 
 template <int KB, bool LDS>
-void launch_find_ob(int ob, const void* keys, int64_t n, const uint32_t* slots, const uint64_t* skeys,
+void launch_find_ob(int ob, const void* keys, int64_t n, const uint32_t* slots, const void* skeys,
                     int cap_log2, uint64_t miss, void* out, hipStream_t s)
 {
     switch (ob) {
@@ -309,7 +353,7 @@ void launch_find_ob(int ob, const void* keys, int64_t n, const uint32_t* slots, 
 }
 
 template <bool LDS>
-void launch_find_kb(int kb, int ob, const void* keys, int64_t n, const uint32_t* slots, const uint64_t* skeys,
+void launch_find_kb(int kb, int ob, const void* keys, int64_t n, const uint32_t* slots, const void* skeys,
                     int cap_log2, uint64_t miss, void* out, hipStream_t s)
 {
     switch (kb) {
@@ -398,20 +442,24 @@ DevBuf dict_encode(const void* keys, int64_t n, const void* dict, int64_t m, int
     const int64_t cap = (int64_t)1 << cap_log2;
     DevBuf slots((size_t)cap * 4, s);
     fill_zero(slots.get(), (size_t)cap * 4, s);
-    DevBuf skeys((size_t)cap * 8, s);
+    const bool in_lds = cap <= kLdsSlotsMax;
+    DevBuf aux((size_t)cap * (in_lds ? 8 : 16), s);
     if (m > 0) {
         hipLaunchKernelGGL(dict_build_kernel, dim3((unsigned)small_grid(m)), dim3(256), 0, s, dict, kb, m,
                            slots.get<uint32_t>(), cap_log2);
         EG_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(dict_slot_keys_kernel, dim3((unsigned)small_grid(cap)), dim3(256), 0, s, dict, kb,
-                       (const uint32_t*)slots.get<uint32_t>(), cap, skeys.get<uint64_t>());
-    EG_LAUNCH_CHECK();
-    if (cap <= kLdsSlotsMax)
-        launch_find_kb<true>(kb, ob, keys, n, slots.get<uint32_t>(), skeys.get<uint64_t>(), cap_log2, miss, out.get(), s);
-    else
-        launch_find_kb<false>(kb, ob, keys, n, slots.get<uint32_t>(), skeys.get<uint64_t>(), cap_log2, miss, out.get(),
-                              s);
+    if (in_lds) {
+        hipLaunchKernelGGL(dict_slot_keys_kernel, dim3((unsigned)small_grid(cap)), dim3(256), 0, s, dict, kb,
+                           (const uint32_t*)slots.get<uint32_t>(), cap, aux.get<uint64_t>());
+        EG_LAUNCH_CHECK();
+        launch_find_kb<true>(kb, ob, keys, n, slots.get<uint32_t>(), aux.get(), cap_log2, miss, out.get(), s);
+    } else {
+        hipLaunchKernelGGL(dict_pack_kernel, dim3((unsigned)small_grid(cap)), dim3(256), 0, s, dict, kb,
+                           (const uint32_t*)slots.get<uint32_t>(), cap, aux.get<uint4>());
+        EG_LAUNCH_CHECK();
+        launch_find_kb<false>(kb, ob, keys, n, slots.get<uint32_t>(), aux.get(), cap_log2, miss, out.get(), s);
+    }
     EG_HIP(hipStreamSynchronize(s));  // the tables are freed on return
     return out;
 }
