@@ -78,6 +78,24 @@ def test_multi_column_key_requests_rejected_before_device_work():
     assert e.value.status == G.EG_ERR_INVALID
 
 
+def test_dictionary_encode_arguments_rejected_before_device_work():
+    """compress/dictionary_encode*.cu: integral key columns only; the
+    dictionary-ready form encodes to INT32 only and needs a dictionary of the
+    column's type; the self-built form needs an integral encoded type."""
+    def status(fn):
+        with pytest.raises(G.GamaxError) as e:
+            fn()
+        return e.value.status
+    f32 = _cv(t=G.type_id.FLOAT32)
+    assert status(lambda: G.dictionary_encode_with_dict(f32, f32)) == G.EG_ERR_UNSUPPORTED
+    assert status(lambda: G.dictionary_encode_with_dict(_cv(), _cv(), encode_type=G.type_id.INT64)) == \
+        G.EG_ERR_UNSUPPORTED
+    assert status(lambda: G.dictionary_encode_with_dict(_cv(), _cv(t=G.type_id.INT64))) == G.EG_ERR_INVALID
+    assert status(lambda: G.dictionary_encode(f32)) == G.EG_ERR_UNSUPPORTED
+    assert status(lambda: G.dictionary_encode(_cv(), G.type_id.FLOAT64)) == G.EG_ERR_UNSUPPORTED
+    assert status(lambda: G.dictionary_encode_with_dict(_cv(ptr=0), _cv())) == G.EG_ERR_INVALID  # null data
+
+
 def test_no_requests_is_invalid():
     with pytest.raises(G.GamaxError) as e:
         G.groupby([_cv()], [_cv()], [])
