@@ -1,8 +1,8 @@
 """Timing of the dictionary lookup (gamax::compress::dictionary_encode with a
 given dictionary) on the reference's own test sizes
 (dictionary_encode_dict_ready_test.cu:93-96): 268435356 keys (0..n-1) mod D
-shuffled against the dictionary 0..D-1, plus int64 keys and a 2^20-key
-dictionary.  Prints one line per case: wall ms of the whole call (range
+shuffled against the dictionary 0..D-1, plus int64 keys, a 30000-key
+dictionary (the largest direct table) and a 2^20-key one.  Prints one line per case: wall ms of the whole call (range
 reduction, table build, lookup) and the lookup's algorithmic rate
 (key + id bytes per row).
 
@@ -20,7 +20,7 @@ def main():
     a = ap.parse_args()
     n = 268435356
     for d, t in [(1024, G.type_id.INT32), (4096, G.type_id.INT32), (4096, G.type_id.INT64),
-                 (1 << 20, G.type_id.INT32)]:
+                 (30000, G.type_id.INT32), (1 << 20, G.type_id.INT32)]:
         keys = G.gen_keys_uniform(n, d, seed=2, t=t)
         dict_col = G.gen_keys_uniform(d, d, seed=3, t=t)  # a permutation of 0..d-1
         best = float("inf")

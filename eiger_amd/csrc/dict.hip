@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kDictBlock) void dict_find_kernel(const void* __res
 // atomicMax keeps the first position of a repeated key), and a row costs one
 // 4-byte LDS read.  key - min is taken modulo 2^(8 KB), so keys outside
 // [min, max] land above the span without a signed compare.
-constexpr int64_t kDirectMax = 16384;
+constexpr int64_t kDirectMax = 32768;  // 128 KB of LDS: one 1024-thread workgroup per CU
 
 __global__ __launch_bounds__(256) void dict_direct_build_kernel(const void* dict, int kb, int64_t m, uint64_t raw_min,
                                                                 uint64_t wmask, uint32_t* table)

@@ -53,10 +53,10 @@ def test_reference_full_size_cases(n, d):
 
 
 @pytest.mark.parametrize("dt", [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64, np.uint64])
-# (dictionary size, key span / size): a span below 16384 takes the direct-indexed
+# (dictionary size, key span / size): a span below 32768 takes the direct-indexed
 # table, a wider one the hashed table (in LDS up to 8192 slots, else in HBM);
 # the 1- and 2-byte types are always direct
-@pytest.mark.parametrize("m,spread", [(0, 3), (5, 3), (3000, 3), (3000, 4096), (20000, 3)])
+@pytest.mark.parametrize("m,spread", [(0, 3), (5, 3), (3000, 3), (10000, 3), (3000, 4096), (20000, 3)])
 def test_random_dictionaries_against_oracle(dt, m, spread):
     rng = np.random.default_rng(m * 31 + spread + np.dtype(dt).itemsize)
     info = np.iinfo(dt)
