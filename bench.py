@@ -1,6 +1,6 @@
 """Benchmark: single-key group-by throughput on one MI355X.
 
-Workload (the strategy sweep the headline metric is quoted on): 2^28 rows,
+Headline workload (the strategy sweep the metric is quoted on): 2^28 rows,
 int32 keys with distinct count D in {2^4, 2^10, 2^16, 2^22, 2^26}, drawn
 uniform (exact counts, shuffled) or Zipf s in {0.5, 1.0, 1.5}; an int64 value
 uniform in [-2^31, 2^31) and a float32 value uniform in [0, 1); aggregates
@@ -9,18 +9,26 @@ the device and stay resident in HBM; each case passes
 hint.groups_upper_bound = D as the reference's benchmark does
 (benchmarks/groupby/groupby_nvbench.cu:57-59).
 
-One step = one group-by of every case.  value = rows / second over the whole
-sweep (Grows/s = N / t / 1e9, the reference's Elem/s).  Algorithmic bytes per
-case = 16 N + 40 G (each input column read once, each output written once).
+One step = one group-by of every case through the top-level dispatcher.
+value = rows / second over the whole sweep (Grows/s = N / t / 1e9, the
+reference's Elem/s).  Algorithmic bytes per case = 16 N + 40 G (each input
+column read once, each output written once).
+
+Also measured in the same run (each its own key of the result line):
+  sweep[*].ms_by_strategy  every case under forced hash, sort and partition
+  partition_heavy          2^30 int64 rows, 2^28 groups, SUM + COUNT
+  batched                  4096 uneven queries in one persistent launch
+  dispatch                 rule vs learned dispatch: on the sweep, and on a
+                           held-out mixed set of queries drawn like the sweep
+                           and the batched mix, against the exhaustive best
+  cpu_baseline             the oracle's OpenMP group-by on the host's cores
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--strategy auto]
-    --strategy auto   the library's dispatcher (HyperLogLog estimate + rule)
-    --strategy hash|sort|sort_gfur|partition|partition_gfur   one strategy
-    --explore         also time every strategy once per case (stderr + sweep)
 """
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -29,21 +37,20 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 METRIC = "group-by Grows/s @256M rows, %HBM peak; learned-vs-fixed dispatch speedup"
-DS = [1 << 4, 1 << 10, 1 << 16, 1 << 22, 1 << 26]
-DISTS = [("uniform", 0.0), ("zipf", 0.5), ("zipf", 1.0), ("zipf", 1.5)]
-SEED = 42
 STRATS = {"auto": "groupby", "hash": "hash_groupby", "sort": "sort_groupby", "sort_gfur": "sort_groupby_gfur",
           "partition": "partition_groupby", "partition_gfur": "partition_groupby_gfur"}
-# Algorithmic HBM bytes per row of one launch of each timed kernel (int32 key,
-# int64 + float32 values): inputs read once; a radix scatter reads and writes
-# the key and both values once (2 x 16 B).
+# Kernel timers of the library (eg_kernel_timing_*) and the kernel functions
+# rocprofv3 reports for them.
 KERNELS = ("hash_agg", "hash_compact", "hll", "key_range", "radix_hist", "radix_scan", "radix_scatter", "seg_heads",
            "segreduce", "partition_bounds", "partition_agg", "partition_compact", "fill_zero", "mlp_select")
-# timer name -> kernel function name (rocprofv3 reports the function)
 TIMER_KERNEL = {"partition_agg": ("part_agg_kernel", "eg_jit_part_agg"), "segreduce": ("seg_reduce_kernel",),
                 "hash_agg": ("hash_agg_kernel", "eg_jit_hash_agg"),
                 "partition_bounds": ("part_bounds_kernel",), "partition_compact": ("part_compact_kernel",)}
-KERNEL_BYTES_PER_ROW = {"hash_agg": 16, "partition_agg": 16, "segreduce": 16, "radix_scatter": 32,
+# Algorithmic bytes per input row of one launch: the workload's input columns
+# read once (int32 key + int64 + float32 values = 16 B), whatever the kernel
+# itself moves (a radix pass reads and writes them; that intermediate is not
+# counted).  Kernels that read only the key: 4 B.
+KERNEL_BYTES_PER_ROW = {"hash_agg": 16, "partition_agg": 16, "segreduce": 16, "radix_scatter": 16,
                         "radix_hist": 4, "seg_heads": 4, "key_range": 4, "hll": 4, "partition_bounds": 4}
 
 
@@ -58,17 +65,19 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--rows", type=int, default=1 << 28)
     p.add_argument("--strategy", default="auto", choices=list(STRATS))
-    p.add_argument("--explore", action="store_true")
     p.add_argument("--rule-only", action="store_true", help="auto dispatch without the learned model")
-    p.add_argument("--no-batch", action="store_true", help="skip the batched uneven-query workload")
     p.add_argument("--cases", default="all", help="'all' or comma list like 16:uniform,1024:zipf1.0")
-    p.add_argument("--cpu-sample-rows", type=int, default=1 << 22)
+    p.add_argument("--no-strategies", action="store_true", help="skip the forced-strategy times per case")
+    p.add_argument("--no-heavy", action="store_true", help="skip the 2^30-row partition-heavy run")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched uneven-query workload")
+    p.add_argument("--no-dispatch-eval", action="store_true", help="skip the held-out dispatch evaluation")
+    p.add_argument("--dispatch-queries", type=int, default=1000)
     p.add_argument("--no-cpu-baseline", action="store_true")
     return p.parse_args()
 
 
-def case_list(spec):
-    cases = [(d, name, s) for d in DS for (name, s) in DISTS]
+def case_list(W, spec):
+    cases = W.sweep_cases()
     if spec == "all":
         return cases
     want = []
@@ -79,11 +88,6 @@ def case_list(spec):
     return want
 
 
-def case_name(c):
-    d, name, s = c
-    return f"D=2^{d.bit_length() - 1} {name}" + ("" if name == "uniform" else f" s={s}")
-
-
 def main():
     args = parse()
     if args.gpus != 1:
@@ -92,25 +96,19 @@ def main():
         sys.exit(2)
 
     from eiger_amd import gamax as G
+    from eiger_amd import workloads as W
 
     A = G.aggregation_id
     n = args.rows
-    cases = case_list(args.cases)
+    cases = case_list(W, args.cases)
     fn = getattr(G, STRATS[args.strategy])
+    reqs = W.sweep_requests(A)
 
     # ---- inputs resident in HBM -------------------------------------------------
     t0 = time.time()
-    v64 = G.gen_values(n, G.val_dist.UNIFORM_I64, G.type_id.INT64, seed=SEED, stream_id=2)
-    vf = G.gen_values(n, G.val_dist.UNIFORM_F32, G.type_id.FLOAT32, seed=SEED, stream_id=3)
-    keys = []
-    for c in cases:
-        d, name, s = c
-        if name == "uniform":
-            keys.append(G.gen_keys_uniform(n, d, seed=SEED, t=G.type_id.INT32))
-        else:
-            keys.append(G.gen_keys_zipf(n, d, s, seed=SEED, t=G.type_id.INT32))
+    v64, vf = W.sweep_values(G, n)
+    keys = [W.sweep_keys(G, d, s, n) for d, _, s in cases]
     log(f"generated {len(cases)} cases x {n} rows in {time.time() - t0:.1f}s")
-    reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MIN), (0, A.MAX), (1, A.SUM)]
 
     def run_case(i, f=fn):
         h = G.groupby_hint(groups_upper_bound=cases[i][0])
@@ -124,134 +122,113 @@ def main():
     copy_gbs = 2 * (1 << 32) / (copy_ms * 1e-3) / 1e9
     log(f"copy probe: {copy_gbs:.0f} GB/s (read+write of 4 GiB)")
 
-    # ---- warmup + timed steps ----------------------------------------------------
+    # ---- specialised kernels: compiled before anything is timed -------------------
+    T = G.type_id
+    if G.jit_stats()["mode"] != G.jit_mode.OFF:
+        G.jit_prefetch(T.INT32, [T.INT64, T.FLOAT32], reqs)
+        G.jit_prefetch(T.INT32, [T.INT64], [(0, A.SUM), (-1, A.COUNT_ALL)])
+        G.jit_prefetch(T.INT64, [T.INT64], [(0, A.SUM), (-1, A.COUNT_ALL)])
+        G.jit_wait()
+    jit0 = G.jit_stats()
+    log(f"specialised kernels: {jit0['compiled']} compiled, {jit0['failed']} failed")
+
+    # ---- warmup ----------------------------------------------------------------
     groups = [0] * len(cases)
-    for w in range(args.warmup):
+    for _ in range(max(args.warmup, 1)):
         for i in range(len(cases)):
             groups[i] = run_case(i)
-    # The aggregation kernels specialised for this request list compile in the
-    # background on first use; let both (hash and partition) finish and load
-    # before anything is timed.
-    jit0 = G.jit_stats()
-    if jit0["mode"] != G.jit_mode.OFF:
-        for f in (G.hash_groupby, G.partition_groupby):
-            run_case(0, f)
-        G.jit_wait()
-        for i in range(len(cases)):
-            run_case(i)
-        for f in (G.hash_groupby, G.partition_groupby):
-            run_case(0, f)
-    jit_warm = G.jit_stats()
-    log(f"specialised kernels: {jit_warm['compiled']} compiled, {jit_warm['failed']} failed")
-    explore = {}
-    if args.explore:
-        for name in ("hash", "sort", "sort_gfur", "partition", "partition_gfur"):
-            f = getattr(G, STRATS[name])
-            for i in range(len(cases)):
-                run_case(i, f)  # the operator returns after its stream is synchronised
-                a = time.perf_counter()
-                run_case(i, f)
-                explore.setdefault(i, {})[name] = round((time.perf_counter() - a) * 1e3, 3)
-            log(f"explored {name}")
-    # ---- dispatch: the learned model (when shipped) against the reference rule ------
+
+    # ---- dispatch on the sweep: the reference rule, then the learned model -------
     dispatch = None
     learned = args.strategy == "auto" and not args.rule_only and G.load_dispatch_model()
     if args.strategy == "auto":
         G.mlp_load(None)  # the rule of auto_select_groupby_algo
         for i in range(len(cases)):
             run_case(i)
-        rule_t = time.perf_counter()
-        for k in range(args.steps):
+        a = time.perf_counter()
+        for _ in range(args.steps):
             for i in range(len(cases)):
                 run_case(i)
-        rule_elapsed = time.perf_counter() - rule_t
-        rule_value = n * len(cases) * args.steps / rule_elapsed / 1e9
+        rule_value = n * len(cases) * args.steps / (time.perf_counter() - a) / 1e9
         log(f"rule dispatch: {rule_value:.3f} Grows/s")
         if learned:
             G.load_dispatch_model()
             for i in range(len(cases)):
                 run_case(i)
-        dispatch = {"rule_grows_per_s": round(rule_value, 3), "learned": bool(learned)}
+        dispatch = {"sweep": {"rule_grows_per_s": round(rule_value, 3), "learned": bool(learned)}}
 
+    # ---- timed steps (the headline) ------------------------------------------------
     G.kernel_timing_reset()
     G.kernel_timing(True)
     jit_timed0 = G.jit_stats()["launches"]
-    per_case = [0.0] * len(cases)
+    per_case = [[] for _ in cases]
     t_start = time.perf_counter()
     for k in range(args.steps):
         for i in range(len(cases)):
             a = time.perf_counter()
             groups[i] = run_case(i)
-            per_case[i] += time.perf_counter() - a
+            per_case[i].append(time.perf_counter() - a)
         log(f"step {k + 1}/{args.steps} done")
     elapsed = time.perf_counter() - t_start
     G.kernel_timing(False)
     jit_end = G.jit_stats()
+    value = n * len(cases) * args.steps / elapsed / 1e9
     if dispatch is not None:
-        v = n * len(cases) * args.steps / elapsed / 1e9
-        dispatch["learned_grows_per_s" if learned else "timed_grows_per_s"] = round(v, 3)
+        sw = dispatch["sweep"]
+        sw["learned_grows_per_s" if learned else "timed_grows_per_s"] = round(value, 3)
         if learned:
-            dispatch["learned_vs_rule"] = round(v / dispatch["rule_grows_per_s"], 3)
-            ev = os.path.join(ROOT, "eiger_amd", "data", "dispatch_eval.json")
-            if os.path.exists(ev):
-                e = json.load(open(ev)).get("heldout_batched", {})
-                dispatch["heldout_mixed_queries"] = {k: e.get(k) for k in (
-                    "queries", "mlp_over_best", "rule_over_mlp", "mlp_picks_best", "rule_picks_best")}
-
-    rows = n * len(cases) * args.steps
-    value = rows / elapsed / 1e9
+            sw["learned_vs_rule"] = round(value / sw["rule_grows_per_s"], 3)
     alg_bytes = sum(16 * n + 40 * g for g in groups) * args.steps
-    sweep = []
-    for i, c in enumerate(cases):
-        t = per_case[i] / args.steps
-        b = 16 * n + 40 * groups[i]
-        sweep.append({"case": case_name(c), "groups": groups[i], "ms": round(t * 1e3, 3),
-                      "grows_per_s": round(n / t / 1e9, 2), "hbm_frac": round(b / t / PEAK_HBM, 4)})
-        if i in explore:
-            sweep[-1]["ms_by_strategy"] = explore[i]
-        log(f"  {case_name(c):24s} G={groups[i]:>9d}  {t * 1e3:8.3f} ms  {n / t / 1e9:7.2f} Grows/s  "
-            f"{b / t / PEAK_HBM * 100:5.1f}% HBM" + (f"  {explore[i]}" if i in explore else ""))
 
-    # ---- dominant kernel roofline --------------------------------------------------
     kernels = {}
     for name in KERNELS:
         cnt, ms = G.kernel_time(name)
         if cnt:
             kernels[name] = (cnt, ms)
-    roofline = None
-    if kernels:
-        dom = max(kernels, key=lambda k: kernels[k][1])
-        cnt, ms = kernels[dom]
-        avg_s = ms / cnt * 1e-3
-        per_launch = KERNEL_BYTES_PER_ROW.get(dom, 16) * n  # every launch covers the n rows of one case
-        achieved = per_launch / avg_s / 1e9
-        traffic = None
-        tf = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-        if os.path.exists(tf):
-            # measured HBM bytes per launch of this kernel on this workload
-            # (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by the copy kernel)
-            for kname, rec in json.load(open(tf)).get("kernels", {}).items():
-                if kname.startswith(TIMER_KERNEL.get(dom, (dom + "_kernel",))):
-                    traffic = int(rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"])
-                    break
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9,
-                    "unit": "GB/s", "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": traffic,
-                    "launches": cnt, "avg_ms": round(ms / cnt, 4), "bytes_per_row": KERNEL_BYTES_PER_ROW.get(dom, 16),
-                    "bytes_per_launch": per_launch, "copy_probe_gbs": round(copy_gbs, 1)}
-        log(f"dominant kernel {dom}: {cnt} launches, avg {ms / cnt:.3f} ms, {achieved:.0f} GB/s algorithmic")
-        for k, (c, m) in kernels.items():
-            log(f"  kernel {k:16s} launches={c:4d} total={m:9.3f} ms")
+    roofline = roofline_of(kernels, n, copy_gbs)
 
-    batched = None
-    if not args.no_batch:
-        batched = batched_workload(G)
+    # ---- every case under each forced strategy (not part of the timed steps) --------
+    by_strategy = {}
+    if not args.no_strategies:
+        for name in ("hash", "sort", "partition"):
+            f = getattr(G, STRATS[name])
+            for i in range(len(cases)):
+                run_case(i, f)
+                ts = []
+                for _ in range(2):
+                    a = time.perf_counter()
+                    run_case(i, f)
+                    ts.append(time.perf_counter() - a)
+                by_strategy.setdefault(i, {})[name] = round(min(ts) * 1e3, 3)
+            log(f"forced {name}: done")
+
+    sweep = []
+    for i, c in enumerate(cases):
+        t = statistics.median(per_case[i])
+        b = 16 * n + 40 * groups[i]
+        rec = {"case": W.case_name(c), "groups": groups[i], "ms": round(t * 1e3, 3),
+               "grows_per_s": round(n / t / 1e9, 2), "hbm_frac": round(b / t / PEAK_HBM, 4)}
+        if i in by_strategy:
+            rec["ms_by_strategy"] = by_strategy[i]
+            best = min(by_strategy[i].values())
+            rec["auto_over_best_forced"] = round(t * 1e3 / best, 3)
+        sweep.append(rec)
+        log(f"  {W.case_name(c):24s} G={groups[i]:>9d}  {t * 1e3:8.3f} ms  {n / t / 1e9:7.2f} Grows/s  "
+            f"{b / t / PEAK_HBM * 100:5.1f}% HBM  {by_strategy.get(i, '')}")
+    for k in (keys):
+        k.free()
+    v64.free()
+    vf.free()
+
+    heavy = None if args.no_heavy else partition_heavy(G)
+    batched = None if args.no_batch else batched_workload(G, W)
+    if batched:
         log(f"batched: {batched['grows_per_s']} Grows/s, {batched['ms']} ms for {batched['queries']} queries "
             f"({batched['speedup_vs_sequential']}x the sequential calls)")
+    if dispatch is not None and learned and not args.no_dispatch_eval:
+        dispatch["heldout_mixed"] = dispatch_eval(G, args.dispatch_queries)
 
-    # ---- CPU baseline (oracle, OpenMP) on a bounded sample -------------------------
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(cases, n, args.cpu_sample_rows)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(W, n)
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Grows/s", "n_gpus": 1,
@@ -262,7 +239,7 @@ def main():
                                "{2^4,2^10,2^16,2^22,2^26} x {uniform, zipf 0.5/1.0/1.5}; COUNT, "
                                "SUM/MIN/MAX(int64), SUM(float32)",
                    "rows": n, "cases": len(cases),
-                   "strategy": ("auto: learned dispatch" if dispatch and dispatch.get("learned") else
+                   "strategy": ("auto: learned dispatch" if learned else
                                 "auto: rule dispatch" if args.strategy == "auto" else args.strategy),
                    "hbm_frac_algorithmic": round(alg_bytes / elapsed / PEAK_HBM, 4),
                    "specialised_kernels": {"mode": ["off", "async", "sync"][jit_end["mode"]],
@@ -270,113 +247,233 @@ def main():
                                            "launches_timed": jit_end["launches"] - jit_timed0}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "kernel_launches": {k: {"launches": c, "total_ms": round(m, 3), "avg_ms": round(m / c, 4)}
+                            for k, (c, m) in kernels.items()},
         "dispatch": dispatch,
+        "partition_heavy": heavy,
         "batched": batched,
         "sweep": sweep,
     }
-    G.jit_wait()  # compiles queued by the batched comparison finish before exit
+    G.jit_wait()
     print(json.dumps(out), flush=True)
 
 
-def batched_workload(G, nq=4096, seed=42):
+def roofline_of(kernels, n, copy_gbs):
+    """The dominant kernel of the timed steps: algorithmic bytes per launch
+    (the rows' 16 input bytes, or 4 for key-only kernels) / mean launch time,
+    and the HBM traffic per launch that rocprofv3 counters measured for it."""
+    if not kernels:
+        return None
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    cnt, ms = kernels[dom]
+    avg_s = ms / cnt * 1e-3
+    bpr = KERNEL_BYTES_PER_ROW.get(dom, 16)
+    per_launch = bpr * n  # every launch covers the n rows of one case
+    achieved = per_launch / avg_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tf):
+        # measured HBM bytes per launch of this kernel on this workload
+        # (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by the copy kernel)
+        for kname, rec in json.load(open(tf)).get("kernels", {}).items():
+            if kname.startswith(TIMER_KERNEL.get(dom, (dom + "_kernel",))):
+                traffic = int(rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"])
+                break
+    log(f"dominant kernel {dom}: {cnt} launches, avg {ms / cnt:.3f} ms, {achieved:.0f} GB/s algorithmic")
+    for k, (c, m) in kernels.items():
+        log(f"  kernel {k:16s} launches={c:4d} total={m:9.3f} ms")
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s", "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": traffic,
+            "traffic_over_algorithmic": round(traffic / per_launch, 3) if traffic else None,
+            "launches": cnt, "avg_ms": round(ms / cnt, 4), "bytes_per_row": bpr,
+            "bytes_per_launch": per_launch, "copy_probe_gbs": round(copy_gbs, 1)}
+
+
+def partition_heavy(G, n=1 << 30, d=1 << 28, reps=3):
+    """2^30 rows of int64 keys, 2^28 distinct values (each exactly four
+    times), int64 values; SUM + COUNT with hint.groups_upper_bound = 2^28.
+    Algorithmic bytes 16 N + 24 G."""
+    A, T = G.aggregation_id, G.type_id
+    k = G.gen_keys_uniform(n, d, seed=42, t=T.INT64)
+    v = G.gen_values(n, G.val_dist.UNIFORM_I64, T.INT64, seed=42, stream_id=2)
+    reqs = [(0, A.SUM), (-1, A.COUNT_ALL)]
+    times, groups, algo = [], 0, None
+    for r in range(reps + 1):
+        h = G.groupby_hint(groups_upper_bound=d)
+        a = time.perf_counter()
+        t = G.groupby([k], [v], reqs, h)
+        dt = time.perf_counter() - a
+        groups = t.num_rows()
+        t.free()
+        algo = int(h.groupby_algo)
+        if r:
+            times.append(dt)
+    k.free()
+    v.free()
+    t = statistics.median(times)
+    log(f"partition-heavy: {t * 1e3:.1f} ms, {groups} groups")
+    return {"rows": n, "groups": groups, "ms": round(t * 1e3, 2), "grows_per_s": round(n / t / 1e9, 2),
+            "hbm_frac": round((16 * n + 24 * groups) / t / PEAK_HBM, 4), "strategy": algo,
+            "runs_ms": [round(x * 1e3, 2) for x in times]}
+
+
+def batched_workload(G, W, nq=4096):
     """Batched uneven queries: nq independent group-bys (rows log-uniform in
     [1e3, 4e6], distinct count log-uniform in [1, rows/2], int32 keys uniform
     or Zipf s ~ U[0.5, 1.5], int64 values; SUM + COUNT) in one persistent
     launch replayed from a hipGraph, tuned by eg_batch_autotune.  Compared
     with the same queries run one by one through the top-level groupby."""
-    import math
-
     import numpy as np
 
-    A = G.aggregation_id
-    T = G.type_id
-    rng = np.random.default_rng(seed)
-    specs = []
-    for q in range(nq):
-        rows = int(math.exp(rng.uniform(math.log(1e3), math.log(4e6))))
-        d = max(1, int(math.exp(rng.uniform(0.0, math.log(max(rows / 2, 1.0))))))
-        s = 0.0 if rng.random() < 0.5 else float(rng.uniform(0.5, 1.5))
-        specs.append((rows, d, s))
-    total = sum(r for r, _, _ in specs)
+    A, T = G.aggregation_id, G.type_id
+    specs = W.mixed_batch_specs(nq)
     t0 = time.time()
-    vals = G.gen_values(total, G.val_dist.UNIFORM_I64, T.INT64, seed=seed, stream_id=2)
-    keys, outs, queries = [], [], []
-    off = 0
-    for q, (rows, d, s) in enumerate(specs):
-        k = G.gen_keys_uniform(rows, d, seed=seed + q, t=T.INT32) if s == 0.0 else \
-            G.gen_keys_zipf(rows, d, s, seed=seed + q, t=T.INT32)
-        keys.append(k)
-        cap = min(rows, d)
-        ok, o1, o2 = G.make_column(T.INT32, cap), G.make_column(T.INT64, cap), G.make_column(T.INT64, cap)
-        outs += [ok, o1, o2]
-        queries.append(G.batch_query(k, G.column_view(vals.data + off * 8, rows, T.INT64), d, ok, [o1, o2]))
-        off += rows
+    mb = W.MixedBatch(G, specs)
+    total = mb.total
     log(f"batched workload: {nq} queries, {total} rows generated in {time.time() - t0:.1f}s")
     reqs = [(0, A.SUM), (-1, A.COUNT_ALL)]
     cands = [G.batch_tuning(tile_rows_log2=t, lds_slots_log2=c, lds_max_bound_log2=b, blocks_per_cu=w)
              for t in (15, 17) for c in (10, 11) for b in (10, 12) for w in (1, 2)]
-    best, times = G.batch_autotune(queries, T.INT32, T.INT64, reqs, cands, reps=1)
-    plan = G.batch_plan(queries, T.INT32, T.INT64, reqs, tuning=best)
+    best, times = G.batch_autotune(mb.queries, T.INT32, T.INT64, reqs, cands, reps=1)
+    plan = G.batch_plan(mb.queries, T.INT32, T.INT64, reqs, tuning=best)
     plan.graph_launch()
     ng, st = plan.results()
     reps = 5
     a = time.perf_counter()
     for _ in range(reps):
         plan.graph_launch()
-    plan.results()
+    ng, st = plan.results()
     batch_s = (time.perf_counter() - a) / reps
-    # the same queries one by one (top-level groupby, rule dispatch, bound hint)
-    G.mlp_load(None)
+    # the same queries one by one (top-level groupby, bound hint)
     a = time.perf_counter()
     for q, (rows, d, s) in enumerate(specs):
-        t = G.groupby([keys[q]], [queries[q].vals], reqs, G.groupby_hint(groups_upper_bound=d))
+        t = G.groupby([mb.keys[q]], [mb.queries[q].vals], reqs, G.groupby_hint(groups_upper_bound=d))
         t.free()
     seq_s = time.perf_counter() - a
     bad = int(np.count_nonzero(st))
-    for c in keys + outs:
-        c.free()
-    vals.free()
+    mb.free()
     plan.free()
     bt = best
     return {"queries": nq, "rows": total, "grows_per_s": round(total / batch_s / 1e9, 3),
             "ms": round(batch_s * 1e3, 3), "sequential_ms": round(seq_s * 1e3, 1),
             "speedup_vs_sequential": round(seq_s / batch_s, 2), "queries_with_status": bad,
+            "groups": int(ng.sum()),
             "algorithmic_hbm_frac": round((12 * total + 20 * int(ng.sum())) / batch_s / PEAK_HBM, 4),
             "tuning": {"tile_rows_log2": bt.tile_rows_log2, "lds_slots_log2": bt.lds_slots_log2,
                        "lds_max_bound_log2": bt.lds_max_bound_log2, "blocks_per_cu": bt.blocks_per_cu},
             "autotune_candidates": len(cands), "autotune_ms": [round(float(x), 3) for x in times]}
 
 
-def cpu_baseline(cases, n, sample):
-    """The oracle's OpenMP group-by (hash partition + per-partition tables) on
-    the first `sample` rows of every case, same aggregates."""
+def dispatch_eval(G, nq, seed=3, key_seed0=3_000_000):
+    """Held-out mixed queries drawn like the sweep and the batched mix (query
+    seed 3; the shipped model was trained on seeds 1 and 2), each run through
+    the top-level group-by with the learned model, with the reference rule,
+    and under each forced strategy, timed in this run (best of two calls
+    each).  Time = the whole synchronous call, dispatch cost included."""
+    import numpy as np
+
+    from eiger_amd import dispatch_train as D
+
+    specs = D.draw_queries(nq, seed=seed)
+    wl = D.Workload(max(s[1] for s in specs))
+    params = np.fromfile(G.DEFAULT_DISPATCH_MODEL, dtype="<f4")
+    names = ("hash", "sort", "partition")
+    fns = [G.hash_groupby, G.sort_groupby, G.partition_groupby]
+    t = np.zeros((nq, 5))  # hash, sort, partition, mlp, rule
+    picks = np.zeros((nq, 2), np.int64)
+    t0 = time.time()
+
+    def best_of(f, k, vals, reqs, d):
+        ts, algo = [], 0
+        for _ in range(2):
+            h = G.groupby_hint(groups_upper_bound=d)
+            a = time.perf_counter()
+            f([k], vals, reqs, h).free()
+            ts.append(time.perf_counter() - a)
+            algo = int(h.groupby_algo)
+        return min(ts) * 1e3, algo
+
+    idx = {G.groupby_algo.HASH: 0, G.groupby_algo.SORT: 1, G.groupby_algo.PARTITION: 2}
+    for i, spec in enumerate(specs):
+        k, vals, reqs, _, _ = wl.query(spec, key_seed0 + i)
+        d = spec[2]
+        for j, f in enumerate(fns):
+            t[i, j], _ = best_of(f, k, vals, reqs, d)
+        G.mlp_load(params)
+        t[i, 3], a = best_of(G.groupby, k, vals, reqs, d)
+        picks[i, 0] = idx.get(a, -1)
+        G.mlp_load(None)
+        t[i, 4], a = best_of(G.groupby, k, vals, reqs, d)
+        picks[i, 1] = idx.get(a, -1)
+        k.free()
+        if (i + 1) % 250 == 0:
+            log(f"dispatch evaluation: {i + 1}/{nq} queries in {time.time() - t0:.0f}s")
+    G.mlp_load(params)
+    wl.v64.free()
+    wl.vf.free()
+    best = t[:, :3].min(axis=1)
+    arg = t[:, :3].argmin(axis=1)
+    tot = t.sum(axis=0)
+    return {"queries": nq, "query_seed": seed, "best_ms": round(float(best.sum()), 1),
+            "mlp_ms": round(float(tot[3]), 1), "rule_ms": round(float(tot[4]), 1),
+            "fixed_ms": {nm: round(float(tot[j]), 1) for j, nm in enumerate(names)},
+            "mlp_over_best": round(float(tot[3] / best.sum()), 4),
+            "rule_over_mlp": round(float(tot[4] / tot[3]), 4),
+            "rule_over_best": round(float(tot[4] / best.sum()), 4),
+            "mlp_picks_best": round(float((picks[:, 0] == arg).mean()), 4),
+            "rule_picks_best": round(float((picks[:, 1] == arg).mean()), 4),
+            "mlp_within_10pct_per_query": round(float((t[:, 3] <= 1.1 * best).mean()), 4),
+            "seconds": round(time.time() - t0, 1)}
+
+
+def host_cpu():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        mask = len(os.sched_getaffinity(0))
+    except AttributeError:
+        mask = os.cpu_count() or 1
+    return model, os.cpu_count(), mask
+
+
+def cpu_baseline(W, n, ds=(1 << 16, 1 << 26), reps=5):
+    """The oracle's OpenMP group-by (hash partition + per-partition tables)
+    on the full 2^28-row uniform cases at D = 2^16 and 2^26, same inputs and
+    aggregates as the sweep, every thread of this process's CPU set
+    (OMP_PROC_BIND=close); one warm-up run, then the median of `reps`."""
+    model, nproc, threads = host_cpu()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ["OMP_PROC_BIND"] = "close"
     import numpy as np
 
     import oracle as O
 
-    try:
-        threads = min(16, len(os.sched_getaffinity(0)))
-    except AttributeError:
-        threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    sample = min(sample, n)
-    v64 = O.gen_values(sample, O.VAL_UNIFORM_I64, np.int64, seed=SEED, stream_id=2)
-    vf = O.gen_values(sample, O.VAL_UNIFORM_F32, np.float32, seed=SEED, stream_id=3)
+    v64 = O.gen_values(n, O.VAL_UNIFORM_I64, np.int64, seed=W.SEED, stream_id=2)
+    vf = O.gen_values(n, O.VAL_UNIFORM_F32, np.float32, seed=W.SEED, stream_id=3)
     reqs = [(-1, O.COUNT_ALL), (0, O.SUM), (0, O.MIN), (0, O.MAX), (1, O.SUM)]
-    total = 0.0
-    for d, name, s in cases:
-        if name == "uniform":
-            k = O.gen_keys_uniform(n, d, seed=SEED, begin=0, count=sample)
-        else:
-            k = O.gen_keys_zipf(sample, O.zipf_cdf(d, s), seed=SEED)
-        a = time.perf_counter()
-        O.groupby(k, [v64, vf], reqs, parallel=True, nthreads=threads)
-        total += time.perf_counter() - a
-    rows = sample * len(cases)
-    log(f"cpu baseline: {rows} rows in {total:.2f}s on {threads} threads")
-    return {"value": round(rows / total / 1e9, 4), "unit": "Grows/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample} rows of each of the {len(cases)} cases (same generator, same 5 "
-                      f"aggregates), oracle OpenMP hash-partition group-by"}
+    per = {}
+    for d in ds:
+        k = O.gen_keys_uniform(n, d, seed=W.SEED)
+        ts = []
+        for r in range(reps + 1):
+            a = time.perf_counter()
+            O.groupby(k, [v64, vf], reqs, parallel=True, nthreads=threads)
+            if r:
+                ts.append(time.perf_counter() - a)
+        per[f"D=2^{d.bit_length() - 1} uniform"] = round(statistics.median(ts) * 1e3, 1)
+        log(f"cpu baseline D={d}: median {statistics.median(ts):.2f}s over {reps} runs on {threads} threads")
+        del k
+    total_ms = sum(per.values())
+    return {"value": round(n * len(ds) / (total_ms * 1e-3) / 1e9, 4), "unit": "Grows/s", "cores": threads,
+            "kind": "port", "ms_per_case": per, "cpu_model": model, "nproc": nproc,
+            "sample": f"full 2^28-row uniform cases at D = 2^16 and 2^26 (same generator, same 5 aggregates); "
+                      f"oracle OpenMP hash-partition group-by, {threads} threads, 1 warm-up + median of {reps}"}
 
 
 if __name__ == "__main__":

@@ -124,6 +124,7 @@ def _proto(L):
     f("eg_jit_wait", C.c_int, [])
     f("eg_jit_stats", C.c_int, [C.POINTER(i64)])
     f("eg_jit_selftest", C.c_int, [C.c_char_p, C.c_char_p, i64])
+    f("eg_jit_prefetch", C.c_int, [i32, C.POINTER(i32), i64, PAR, i64, C.c_char_p, C.POINTER(i32)])
     f("eg_gen_keys_uniform", C.c_int, [vp, i32, i64, i64, i64, i64, u64, vp])
     f("eg_zipf_cdf", C.c_int, [vp, i64, C.c_double])
     f("eg_gen_keys_zipf", C.c_int, [vp, i32, i64, i64, vp, i64, u64, vp])

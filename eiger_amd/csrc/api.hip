@@ -26,7 +26,7 @@ void zipf_cdf(double* cdf, int64_t distinct, double s);
 Result run_sort(const Plan& plan, const eg_groupby_hint& hint, bool gfur, hipStream_t s);
 Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream_t s);
 bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_bound, int nops, int value_bytes,
-                hipStream_t s, int32_t* algo, int64_t* card_est);
+                hipStream_t s, int32_t* algo, int64_t* card_est, float* singletons);
 int64_t ndv_approx(const void* data, int32_t type, int64_t n, hipStream_t s);
 void scalar_aggregate(const std::vector<ViewIn>& cols, const eg_agg_request* reqs, int64_t nreqs,
                       hipStream_t s, Result& res);
@@ -377,17 +377,26 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
                 for (int c = 0; c < plan.spec.nvcols; ++c) vbytes += plan.spec.cols[c].bytes;
                 int32_t a = 0;
                 int64_t card = 0;
+                float singletons = 1.f;
                 if (mlp_select(plan.key.data, plan.key_type, plan.rows, h.groups_upper_bound, plan.spec.nops, vbytes, s,
-                               &a, &card)) {
+                               &a, &card, &singletons)) {
                     h.groupby_algo = a;
-                    // The model's group-count estimate, with headroom, sizes the
-                    // tables; a caller's bound is an upper bound, often loose
-                    // (skewed keys use few of the possible values), so the
-                    // smaller of the two is taken.  An estimate that is too low
-                    // costs a retry or LDS spills, never a wrong result.
-                    learned_bound = std::min<int64_t>(plan.rows, card + card + card / 2 + 1024);
-                    if (h.groups_upper_bound > 0 && h.groups_upper_bound < learned_bound)
-                        learned_bound = h.groups_upper_bound;
+                    // The tables are sized from a group-count bound.  A caller's
+                    // bound is an upper bound, often loose (skewed keys use few
+                    // of the possible values), so a smaller estimate replaces
+                    // it.  The model's estimate (with headroom) is used when the
+                    // sample has seen most groups (at most 1/4 of its keys seen
+                    // once); otherwise the sample only gives a lower bound and
+                    // the full-column HyperLogLog estimate (standard error 0.8%,
+                    // 25% headroom) sizes the tables, so an undersized table
+                    // and its retry do not happen.
+                    const int64_t caller = (h.groups_upper_bound > 0 && h.groups_upper_bound < plan.rows)
+                                               ? h.groups_upper_bound : plan.rows;
+                    learned_bound = std::min<int64_t>(caller, card + card + card / 2 + 1024);
+                    if (learned_bound < caller && singletons > 0.25f) {
+                        const int64_t est = ndv_approx(plan.key.data, plan.key_type, plan.rows, s);
+                        learned_bound = std::min<int64_t>(caller, est + est / 4 + 1024);
+                    }
                 } else {
                     int rc = eg_auto_select_algo(keys, num_keys, &h, stream);
                     if (rc != EG_OK) raise(rc, last_error_cstr());

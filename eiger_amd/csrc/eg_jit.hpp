@@ -17,6 +17,11 @@ enum class Kind { HASH_AGG, PART_AGG, PART_AGG_GATHER };
 // size threshold, compile pending (async mode) or failed.
 hipFunction_t lookup(Kind kind, int kb, const std::string& vlayout, const AggSpec& spec, int64_t rows);
 
+// Queues the specialisation for `arch` on the background worker without
+// touching a device (lookups on a device of that architecture string find
+// it); false when it is already known, JIT is off, or the process is exiting.
+bool prefetch(Kind kind, int kb, const std::string& vlayout, const AggSpec& spec, const std::string& arch);
+
 // hipModuleLaunchKernel; false (and the entry disabled) when the launch is
 // refused, so the caller launches its generic kernel instead.
 bool launch(hipFunction_t f, unsigned grid, unsigned block, size_t lds, hipStream_t s, void** args);

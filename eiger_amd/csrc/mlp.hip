@@ -36,6 +36,7 @@ namespace eg {
 namespace {
 
 constexpr int kIn = 16, kH = 64, kOut = 4;
+constexpr int kSelOut = kOut + 1;  // select_kernel: the four outputs, then the sample's singleton fraction
 constexpr int kW1 = 0, kB1 = kW1 + kH * kIn, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH,
               kB3 = kW3 + kOut * kH, kParams = kB3 + kOut;  // 5508
 static_assert(kParams == EG_MLP_PARAMS, "parameter layout");
@@ -260,6 +261,7 @@ __global__ __launch_bounds__(kFB) void select_kernel(FQuery q, const float* __re
         for (int v = 0; v < kH; ++v) z = fmaf(P[kW3 + u * kH + v], h2[v], z);
         y[u] = z;
     }
+    if (u == kOut) y[kOut] = x[7];  // the sample's singleton fraction
 }
 
 // ---- MLP kernels (256 threads, 64 samples per workgroup) ---------------------------
@@ -488,7 +490,7 @@ struct LoadedModel {
     std::mutex mu;
     DevBuf params;
     DevBuf out;                 // 4 floats
-    float* host_out = nullptr;  // pinned, 4 floats
+    float* host_out = nullptr;  // pinned, kSelOut floats
     bool loaded = false;
 };
 LoadedModel& loaded_model()
@@ -512,7 +514,7 @@ int algo_of(const float* o, int64_t rows)
 
 // Learned strategy choice for one key column; false when no model is loaded.
 bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_bound, int nops, int value_bytes,
-                hipStream_t s, int32_t* algo, int64_t* card_est)
+                hipStream_t s, int32_t* algo, int64_t* card_est, float* singletons)
 {
     LoadedModel& m = loaded_model();
     std::lock_guard<std::mutex> lk(m.mu);
@@ -524,9 +526,9 @@ bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_
                            m.out.get<float>());
         EG_LAUNCH_CHECK();
     }
-    EG_HIP(hipMemcpyAsync(m.host_out, m.out.get(), kOut * 4, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipMemcpyAsync(m.host_out, m.out.get(), kSelOut * 4, hipMemcpyDeviceToHost, s));
     EG_HIP(hipStreamSynchronize(s));
-    float h[kOut];
+    float h[kSelOut];
     std::memcpy(h, m.host_out, sizeof h);
     for (float v : h)
         if (!std::isfinite(v)) return false;  // fall back to the rule
@@ -535,6 +537,7 @@ bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_
         const double lg = std::min(62.0, std::max(0.0, (double)h[3] * 32.0));
         *card_est = (int64_t)std::min<double>((double)rows, std::exp2(lg));
     }
+    if (singletons) *singletons = h[kOut];
     return true;
 }
 
@@ -678,8 +681,8 @@ int eg_mlp_load(const float* host_params, int64_t n)
             if (!std::isfinite(host_params[i])) eg::raise(EG_ERR_INVALID, "non-finite parameter");
         m.params = eg::DevBuf((size_t)n * 4, nullptr);
         EG_HIP(hipMemcpy(m.params.get(), host_params, (size_t)n * 4, hipMemcpyHostToDevice));
-        m.out = eg::DevBuf(eg::kOut * 4, nullptr);
-        if (!m.host_out) EG_HIP(hipHostMalloc((void**)&m.host_out, eg::kOut * 4, hipHostMallocDefault));
+        m.out = eg::DevBuf(eg::kSelOut * 4, nullptr);
+        if (!m.host_out) EG_HIP(hipHostMalloc((void**)&m.host_out, eg::kSelOut * 4, hipHostMallocDefault));
         m.loaded = true;
     });
 }
@@ -694,7 +697,7 @@ int eg_mlp_select(eg_column_view key, int64_t group_bound, int32_t nops, int32_t
             eg::raise(EG_ERR_UNSUPPORTED, "features need 4- or 8-byte integer keys");
         if (key.size <= 0) eg::raise(EG_ERR_INVALID, "empty key column");
         const void* data = (const char*)key.data + key.offset * kb;
-        if (!eg::mlp_select(data, key.type, key.size, group_bound, nops, value_bytes, (hipStream_t)stream, algo, card_est))
+        if (!eg::mlp_select(data, key.type, key.size, group_bound, nops, value_bytes, (hipStream_t)stream, algo, card_est, nullptr))
             eg::raise(EG_ERR_INVALID, "no dispatch model is loaded");
     });
 }

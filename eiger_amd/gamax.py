@@ -536,6 +536,17 @@ def jit_stats() -> dict:
             "mode": out[4], "min_rows": out[5]}
 
 
+def jit_prefetch(key_type: int, value_types: Sequence[int], requests, arch: Optional[str] = None) -> int:
+    """Queues the hash and partition kernels specialised for this request list
+    on the background compiler (arch None = the current device's); returns the
+    number of new compiles.  Needs no device when `arch` is given."""
+    vt = (C.c_int32 * max(len(value_types), 1))(*[int(t) for t in value_types])
+    q = C.c_int32(0)
+    _check(lib().eg_jit_prefetch(int(key_type), vt, len(value_types), _reqs(requests), len(requests),
+                                 arch.encode() if arch else None, C.byref(q)))
+    return q.value
+
+
 def jit_selftest(arch: str = "gfx950") -> None:
     """Compiles a fixed set of specialised kernels for `arch` (no device
     needed); raises with the compiler log on failure."""

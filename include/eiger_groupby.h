@@ -410,6 +410,17 @@ int eg_jit_wait(void);
 /* out6 = {kernels compiled, compiles or loads failed, compiles pending,
  *         specialised launches, mode, min_rows}. */
 int eg_jit_stats(int64_t* out6);
+/* Queues, on the background compiler thread, the hash and partition
+ * aggregation kernels specialised for this request list (key type, value
+ * column types, requests as eg_groupby lowers them), for `arch` (e.g.
+ * "gfx950:sramecc+:xnack-", the device's gcnArchName; NULL = the current
+ * device's).  Needs no device when `arch` is given.  Lets a caller warm the
+ * specialisation cache before its data arrives; *queued (may be NULL) = the
+ * number of new compiles.  A process may exit with compiles queued or
+ * running. */
+int eg_jit_prefetch(int32_t key_type, const int32_t* val_types, int64_t num_vals,
+                    const eg_agg_request* reqs, int64_t num_reqs, const char* arch,
+                    int32_t* queued);
 /* Diagnostic: compiles a fixed set of specialised kernels (hash and partition,
  * 4- and 8-byte keys, vector and typed-load value layouts) for `arch` (e.g.
  * "gfx950") without a device; the compiler log of failures goes to log. */

@@ -109,6 +109,10 @@ def lib():
         L.orc_mlp_forward.argtypes = [vp, vp, i64, vp]
         L.orc_mlp_loss_grad.restype = C.c_int
         L.orc_mlp_loss_grad.argtypes = [vp, vp, vp, vp, i64, C.c_double, vp, vp]
+        L.orc_argsort_keys.restype = C.c_int
+        L.orc_argsort_keys.argtypes = [vp, i32, i64, i32, vp]
+        L.orc_gather.restype = C.c_int
+        L.orc_gather.argtypes = [vp, i32, vp, i64, vp, i32]
         L.orc_sample_features.restype = C.c_int
         L.orc_sample_features.argtypes = [vp, i32, i64, i64, i32, i32, vp]
         _lib = L
@@ -247,6 +251,40 @@ def sort_by_key(cols):
     """Rows of a group-by result ordered by key (column 0), stable."""
     order = np.argsort(cols[0], kind="stable")
     return [c[order] for c in cols]
+
+
+def _threads(nthreads):
+    if nthreads:
+        return int(nthreads)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def argsort_keys(keys, nthreads=None) -> np.ndarray:
+    """Stable ascending order of an integer key column (parallel LSD radix)."""
+    keys = np.ascontiguousarray(keys)
+    perm_out = np.empty(len(keys), dtype=np.int64)
+    if lib().orc_argsort_keys(_ptr(keys), type_id(keys.dtype), len(keys), _threads(nthreads), _ptr(perm_out)) != 0:
+        raise OracleError(lib().orc_last_error().decode())
+    return perm_out
+
+
+def take(a, idx, nthreads=None) -> np.ndarray:
+    """a[idx] (parallel gather)."""
+    a = np.ascontiguousarray(a)
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    out = np.empty(len(idx), dtype=a.dtype)
+    if lib().orc_gather(_ptr(a), a.dtype.itemsize, _ptr(idx), len(idx), _ptr(out), _threads(nthreads)) != 0:
+        raise OracleError(lib().orc_last_error().decode())
+    return out
+
+
+def sort_by_key_fast(cols, nthreads=None):
+    """sort_by_key for large results (parallel radix order + gathers)."""
+    order = argsort_keys(cols[0], nthreads)
+    return [take(c, order, nthreads) for c in cols]
 
 
 # ---------------------------------------------------------------- sort / partition

@@ -1018,3 +1018,92 @@ int64_t orc_dict_encode_ordered(const void* keys, int32_t type, int64_t n, int64
     free(t);
     return d + 1;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Checker helpers for full-size results (no reference counterpart: the     */
+/* reference's tests compare group-by results after sorting them by key,    */
+/* floating_point_test.cu:16).                                              */
+/* ------------------------------------------------------------------------ */
+
+/* perm = the row order that sorts keys ascending (signed keys by value,
+ * unsigned by value), stable: an OpenMP LSD radix sort on 8-bit digits of the
+ * order-preserving 64-bit key, passes whose digit is the same for every row
+ * skipped.  Returns 0, or -1 when memory runs out. */
+int orc_argsort_keys(const void* keys, int32_t type, int64_t n, int32_t nthreads, int64_t* perm)
+{
+    if (n <= 0) return 0;
+    if (!key_type_ok(type)) return fail("argsort: unsupported key type");
+    int T = nthreads > 0 ? nthreads : 1;
+#ifndef _OPENMP
+    T = 1;
+#endif
+    uint64_t* ka = (uint64_t*)malloc((size_t)n * 8);
+    uint64_t* kb = (uint64_t*)malloc((size_t)n * 8);
+    int64_t* pb = (int64_t*)malloc((size_t)n * 8);
+    int64_t* hist = (int64_t*)malloc((size_t)T * 256 * 8);
+    if (!ka || !kb || !pb || !hist) {
+        free(ka); free(kb); free(pb); free(hist);
+        return fail("out of memory");
+    }
+    int64_t* pa = perm;
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        ka[i] = key_ord(keys, type, i);
+        pa[i] = i;
+    }
+    for (int shift = 0; shift < 64; shift += 8) {
+        int skip = 0;
+#pragma omp parallel num_threads(T)
+        {
+            int t = 0;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+#endif
+            const int64_t lo = n * t / T, hi = n * (t + 1) / T;
+            int64_t* h = hist + (size_t)t * 256;
+            for (int d = 0; d < 256; ++d) h[d] = 0;
+            for (int64_t i = lo; i < hi; ++i) h[(ka[i] >> shift) & 255]++;
+#pragma omp barrier
+#pragma omp single
+            {
+                int64_t acc = 0;
+                for (int d = 0; d < 256; ++d) {
+                    int64_t tot = 0;
+                    for (int u = 0; u < T; ++u) tot += hist[(size_t)u * 256 + d];
+                    if (tot == n) skip = 1;
+                    for (int u = 0; u < T; ++u) {
+                        const int64_t c = hist[(size_t)u * 256 + d];
+                        hist[(size_t)u * 256 + d] = acc;
+                        acc += c;
+                    }
+                }
+            }
+            if (!skip)
+                for (int64_t i = lo; i < hi; ++i) {
+                    const int64_t o = h[(ka[i] >> shift) & 255]++;
+                    kb[o] = ka[i];
+                    pb[o] = pa[i];
+                }
+        }
+        if (skip) continue;
+        uint64_t* tk = ka; ka = kb; kb = tk;
+        int64_t* tp = pa; pa = pb; pb = tp;
+    }
+    if (pa != perm) {
+        memcpy(perm, pa, (size_t)n * 8);
+        pb = pa;
+    }
+    free(ka); free(kb); free(pb); free(hist);
+    return 0;
+}
+
+/* dst[i] = src[idx[i]] for elements of `bytes` bytes (1, 2, 4 or 8). */
+int orc_gather(const void* src, int32_t bytes, const int64_t* idx, int64_t n, void* dst, int32_t nthreads)
+{
+    int T = nthreads > 0 ? nthreads : 1;
+    if (bytes != 1 && bytes != 2 && bytes != 4 && bytes != 8) return fail("gather: element size");
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (int64_t i = 0; i < n; ++i)
+        memcpy((char*)dst + i * bytes, (const char*)src + idx[i] * bytes, (size_t)bytes);
+    return 0;
+}

@@ -122,6 +122,10 @@ int orc_dict_encode(const void* keys, const void* dict, int32_t type, int64_t n,
                     int64_t* out);
 int64_t orc_dict_encode_ordered(const void* keys, int32_t type, int64_t n, int64_t* out);
 
+/* ---- checker helpers (OpenMP) --------------------------------------------- */
+int orc_argsort_keys(const void* keys, int32_t type, int64_t n, int32_t nthreads, int64_t* perm);
+int orc_gather(const void* src, int32_t bytes, const int64_t* idx, int64_t n, void* dst, int32_t nthreads);
+
 /* ---- learned dispatch (eg_oracle_mlp.c): fp64 restatement ---------------- */
 #define ORC_MLP_PARAMS 5508
 int orc_mlp_forward(const double* params, const float* x, int64_t n, double* y);

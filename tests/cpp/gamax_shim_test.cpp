@@ -307,10 +307,27 @@ static void test_phys_wrappers()
     pass("phys_wrappers");
 }
 
+// Output directory for arrays the Python side checks against the oracle
+// (argv[1]; none: nothing is written).
+static const char* g_dump_dir = nullptr;
+
+template <class T>
+static void dump(const char* name, const std::vector<T>& v)
+{
+    if (!g_dump_dir) return;
+    const std::string path = std::string(g_dump_dir) + "/" + name + ".bin";
+    FILE* f = std::fopen(path.c_str(), "wb");
+    CHECK(f != nullptr);
+    if (!f) return;
+    CHECK(std::fwrite(v.data(), sizeof(T), v.size(), f) == v.size());
+    std::fclose(f);
+}
+
 // The pull-based GroupBy operator (designed here; the reference snapshot has
 // no GroupBy pullop): a scan in 100000-row chunks under a GroupByOperator
 // emitting 1000-row chunks gives the same groups as one groupby call; an
-// empty input gives no chunk.
+// empty input gives no chunk.  The input and the emitted rows are also
+// written out (g_dump_dir) and compared with the oracle by the Python test.
 static void test_pullops_group_by()
 {
     const int64_t n = 1000000;
@@ -332,6 +349,8 @@ static void test_pullops_group_by()
     pullops::GroupByOperator gb(&scan, 1, reqs, groupby_hint{}, 1000);
     gb.open();
     std::map<int32_t, std::tuple<int64_t, int64_t, int64_t>> got;
+    std::vector<int32_t> ok;
+    std::vector<int64_t> os, oc, om;
     int chunks = 0;
     for (auto c = gb.next(); !c.is_end(); c = gb.next()) {
         CHECK(c.num_items <= 1000 && c.data->num_columns() == 4);
@@ -343,10 +362,20 @@ static void test_pullops_group_by()
             CHECK(got.count(k[i]) == 0);
             got[k[i]] = std::make_tuple(s[i], n1[i], mx[i]);
         }
+        ok.insert(ok.end(), k.begin(), k.end());
+        os.insert(os.end(), s.begin(), s.end());
+        oc.insert(oc.end(), n1.begin(), n1.end());
+        om.insert(om.end(), mx.begin(), mx.end());
         c.free_mem();
         ++chunks;
     }
     gb.close();
+    dump("pullops_in_keys", hk);
+    dump("pullops_in_vals", hv);
+    dump("pullops_out_keys", ok);
+    dump("pullops_out_sum", os);
+    dump("pullops_out_count", oc);
+    dump("pullops_out_max", om);
     CHECK(gb.hint_used().groupby_algo != groupby::AUTO_ALGO);
 
     std::vector<size_type> k0{0}, v0{1};
@@ -396,8 +425,9 @@ static void test_errors()
     pass("errors");
 }
 
-int main()
+int main(int argc, char** argv)
 {
+    if (argc > 1) g_dump_dir = argv[1];
     for (auto [name, algo] : kAlgos) {
         test_uniform_closed_form(algo, name, 1 << 20, 1024);
         test_uniform_closed_form(algo, name, 1 << 20, 4);

@@ -496,3 +496,28 @@ def test_dict_encode_ordered_matches_numpy_unique(dt):
     # the ordered ids are the positions in the sorted distinct keys
     np.testing.assert_array_equal(O.dict_encode(keys, uniq), ids)
     assert O.dict_encode_ordered(keys[:0])[1] == 0
+
+
+# ---- checker helpers for full-size results ------------------------------------------
+@pytest.mark.parametrize("dt", [np.int8, np.int16, np.int32, np.int64, np.uint8, np.uint16, np.uint32, np.uint64])
+def test_argsort_keys_is_numpy_stable_argsort(dt):
+    rng = np.random.default_rng(21)
+    info = np.iinfo(dt)
+    k = rng.integers(info.min, info.max, 50_001, dtype=dt, endpoint=True)
+    k[:500] = k[500:1000]  # repeated keys: stability matters
+    k[1000:1004] = [info.min, info.max, 0, info.min]
+    for nt in (1, 3):
+        np.testing.assert_array_equal(O.argsort_keys(k, nthreads=nt), np.argsort(k, kind="stable"))
+    assert len(O.argsort_keys(k[:0])) == 0
+
+
+def test_take_and_sort_by_key_fast():
+    rng = np.random.default_rng(22)
+    cols = [rng.permutation(100_000).astype(np.int64) - 50_000, rng.random(100_000).astype(np.float32),
+            rng.integers(0, 9, 100_000).astype(np.int16)]
+    idx = rng.integers(0, 100_000, 70_000)
+    for c in cols:
+        np.testing.assert_array_equal(O.take(c, idx), c[idx])
+    fast, slow = O.sort_by_key_fast(cols), O.sort_by_key(cols)
+    for a, b in zip(fast, slow):
+        np.testing.assert_array_equal(a, b)
