@@ -43,6 +43,9 @@ STRATS = {"auto": "groupby", "hash": "hash_groupby", "sort": "sort_groupby", "so
 # rocprofv3 reports for them.
 KERNELS = ("hash_agg", "hash_compact", "hll", "key_range", "radix_hist", "radix_scan", "radix_scatter", "seg_heads",
            "segreduce", "partition_bounds", "partition_agg", "partition_compact", "fill_zero", "mlp_select")
+# Retries counted by the library (a table or output sized too small for the
+# groups found: that strategy's work is redone inside the call).
+RETRIES = ("hash_retry", "partition_retry_table", "partition_retry_capacity")
 TIMER_KERNEL = {"partition_agg": ("part_agg_kernel", "eg_jit_part_agg"), "segreduce": ("seg_reduce_kernel",),
                 "hash_agg": ("hash_agg_kernel", "eg_jit_hash_agg"),
                 "partition_bounds": ("part_bounds_kernel",), "partition_compact": ("part_compact_kernel",)}
@@ -186,6 +189,7 @@ def main():
         if cnt:
             kernels[name] = (cnt, ms)
     roofline = roofline_of(kernels, n, copy_gbs)
+    retries = {name: G.kernel_time(name)[0] for name in RETRIES}
 
     # ---- every case under each forced strategy (not part of the timed steps) --------
     by_strategy = {}
@@ -249,6 +253,7 @@ def main():
         "cpu_baseline": cpu,
         "kernel_launches": {k: {"launches": c, "total_ms": round(m, 3), "avg_ms": round(m / c, 4)}
                             for k, (c, m) in kernels.items()},
+        "retries_timed": retries,
         "dispatch": dispatch,
         "partition_heavy": heavy,
         "batched": batched,

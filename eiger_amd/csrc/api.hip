@@ -176,31 +176,39 @@ Plan make_plan(const ViewIn* key, const std::vector<ViewIn>& vals, const eg_agg_
     return b.p;
 }
 
+void set_outspec(const Plan& plan, void* key_data, void* const* out_data, OutSpec& os)
+{
+    std::memset(&os, 0, sizeof os);
+    os.key_data = key_data;
+    os.key_bytes = plan.key.bytes;
+    os.nouts = (int32_t)plan.outs.size();
+    for (size_t j = 0; j < plan.outs.size(); ++j) {
+        const OutCol& o = plan.outs[j];
+        OutDesc& d = os.outs[j];
+        d.data = out_data[j];
+        d.code = o.code;
+        d.bytes = type_size(o.type);
+        d.op = o.op;
+        d.op2 = o.op2;
+        d.is_float = is_float(o.type) ? 1 : 0;
+    }
+}
+
 void alloc_outputs(const Plan& plan, int64_t groups, hipStream_t s, Result& res, OutSpec& os)
 {
     res.num_groups = groups;
     res.cols.clear();
     res.types.clear();
-    std::memset(&os, 0, sizeof os);
     const size_t rows = (size_t)std::max<int64_t>(groups, 0);
     res.cols.emplace_back(rows * plan.key.bytes, s);
     res.types.push_back(plan.key_type);
-    os.key_data = res.cols.back().get();
-    os.key_bytes = plan.key.bytes;
-    os.nouts = (int32_t)plan.outs.size();
+    void* outs[kMaxOuts] = {};
     for (size_t j = 0; j < plan.outs.size(); ++j) {
-        const OutCol& o = plan.outs[j];
-        const int bytes = type_size(o.type);
-        res.cols.emplace_back(rows * bytes, s);
-        res.types.push_back(o.type);
-        OutDesc& d = os.outs[j];
-        d.data = res.cols.back().get();
-        d.code = o.code;
-        d.bytes = bytes;
-        d.op = o.op;
-        d.op2 = o.op2;
-        d.is_float = is_float(o.type) ? 1 : 0;
+        res.cols.emplace_back(rows * type_size(plan.outs[j].type), s);
+        res.types.push_back(plan.outs[j].type);
+        outs[j] = res.cols.back().get();
     }
+    set_outspec(plan, res.cols[0].get(), outs, os);
 }
 
 }  // namespace eg
@@ -352,15 +360,49 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
         std::vector<ViewIn> vs;
         for (int64_t i = 0; i < num_vals; ++i) vs.push_back(view_in(vals[i]));
         const bool top = entry == Entry::TOP;
+        // 1- and 2-byte integer, BOOL8 and float keys run on a mapped 4- or
+        // 8-byte integer key (keynorm.hip); the group keys are mapped back.
+        // Partition takes integral keys only (groupby_partition.cu:556-567).
+        const int32_t orig_key_type = k.type;
+        const bool mapped_key = key_needs_mapping(orig_key_type);
+        if (mapped_key) {
+            const bool part = entry == Entry::PARTITION || entry == Entry::PARTITION_GFUR ||
+                              (top && (h.groupby_algo == EG_ALGO_PARTITION || h.groupby_algo == EG_ALGO_PARTITION_GFUR));
+            if (part && is_float(orig_key_type))
+                raise(EG_ERR_UNSUPPORTED, "Non-integral key type is not supported for partition-based groupby.");
+            k.type = mapped_key_type(orig_key_type);
+        }
+        DevBuf mapped;
+        eg_column_view mapped_view{};
+        const eg_column_view* kviews = keys;
+        auto map_key_column = [&]() {
+            if (!mapped_key) return;
+            if (k.size < 0) raise(EG_ERR_INVALID, "negative key column size");
+            if (k.size > 0 && !k.data) raise(EG_ERR_INVALID, "null key column data");
+            mapped = map_keys(k.data, orig_key_type, k.size, s);
+            k.data = mapped.get();
+            mapped_view = eg_column_view{mapped.get(), k.size, k.type, 0, 0};
+            kviews = &mapped_view;
+        };
+        auto finish = [&](Result& res) {
+            if (mapped_key && !res.cols.empty()) {
+                res.cols[0] = unmap_keys(res.cols[0].get(), res.num_groups, orig_key_type, s);
+                res.types[0] = orig_key_type;
+            }
+            EG_HIP(hipStreamSynchronize(s));
+            *out = to_table(res);
+        };
         Plan plan;
         try {
             plan = make_plan(&k, vs, reqs, num_reqs, top && h.agg_count != 0, top);
         } catch (const WideError&) {
+            map_key_column();
             Result res = wide_groupby(k, vs, reqs, num_reqs, top, h.agg_count != 0, h, s);
-            EG_HIP(hipStreamSynchronize(s));
-            *out = to_table(res);
+            finish(res);
             return;
         }
+        map_key_column();
+        plan.key.data = k.data;
 
         int algo;
         int64_t learned_bound = 0;  // group bound from the dispatch model's estimate
@@ -398,7 +440,7 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
                         learned_bound = std::min<int64_t>(caller, est + est / 4 + 1024);
                     }
                 } else {
-                    int rc = eg_auto_select_algo(keys, num_keys, &h, stream);
+                    int rc = eg_auto_select_algo(kviews, num_keys, &h, stream);
                     if (rc != EG_OK) raise(rc, last_error_cstr());
                 }
             }
@@ -425,8 +467,7 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
             default: raise(EG_ERR_UNSUPPORTED, "Unsupported groupby algorithm");
             }
         }
-        EG_HIP(hipStreamSynchronize(s));
-        *out = to_table(res);
+        finish(res);
     });
 }
 }  // namespace

@@ -364,25 +364,6 @@ void launch_find_kb(int kb, int ob, const void* keys, int64_t n, const uint32_t*
     }
 }
 
-// 1- and 2-byte keys are widened to INT32 for the distinct-key pass (the
-// group-by takes 4- and 8-byte keys); widening keeps order and equality.
-__global__ __launch_bounds__(256) void widen_kernel(const void* in, int kb, int sign, int64_t n, int32_t* out)
-{
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t r = (uint32_t)dev::load_raw(in, kb, i);
-        const int sh = 32 - 8 * kb;
-        out[i] = sign ? ((int32_t)(r << sh) >> sh) : (int32_t)r;
-    }
-}
-
-__global__ __launch_bounds__(256) void narrow_kernel(const int32_t* in, int kb, int64_t n, void* out)
-{
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        dev::store_bytes(out, kb, i, (uint64_t)(uint32_t)in[i]);
-}
-
 int64_t small_grid(int64_t n)
 {
     return std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)device_info().num_cus * 8));
@@ -395,9 +376,12 @@ const void* view_ptr(const eg_column_view& c, int bytes)
     return c.data ? (const char*)c.data + c.offset * bytes : nullptr;
 }
 
-void check_key_type(int32_t t)
+// The dict-ready form takes integral keys (dictionary_encode_dict_ready.cu);
+// the self-built forms also take floats (dictionary_encode.cu type-dispatches
+// every non-string type).
+void check_key_type(int32_t t, bool floats_ok)
 {
-    if (!is_int(t) && t != EG_BOOL8)
+    if (!is_int(t) && t != EG_BOOL8 && !(floats_ok && is_float(t)))
         raise(EG_ERR_UNSUPPORTED, "Dictionary encoding is not supported for non-integral type.");
 }
 
@@ -474,7 +458,7 @@ int eg_dictionary_encode_with_dict(eg_column_view column, eg_column_view dict, i
     return guard([&] {
         if (!out) raise(EG_ERR_INVALID, "null output");
         if (encode_type != EG_INT32) raise(EG_ERR_UNSUPPORTED, "Unsupported encoded type");
-        check_key_type(column.type);
+        check_key_type(column.type, false);
         if (dict.type != column.type) raise(EG_ERR_INVALID, "dictionary and column types differ");
         const int kb = type_size(column.type);
         const void* keys = view_ptr(column, kb);
@@ -496,25 +480,23 @@ int eg_dictionary_encode(eg_column_view column, int32_t encode_type, int64_t dic
     const int rc = guard([&] {
         if (!out || !out_dict_size) raise(EG_ERR_INVALID, "null output");
         if (!is_int(encode_type)) raise(EG_ERR_UNSUPPORTED, "Encoded type must be integral.");
-        check_key_type(column.type);
-        int kb = type_size(column.type);
+        check_key_type(column.type, true);
+        const int kb = type_size(column.type);
         const void* keys = view_ptr(column, kb);
         hipStream_t s = (hipStream_t)stream;
-        const int orig_kb = kb;
+        // 1- and 2-byte integer, BOOL8 and float keys: the distinct-key pass
+        // and the lookup run on the mapped 4- or 8-byte integer keys (equal
+        // keys <=> equal mapped keys, order kept; floats: -0.0 = +0.0, one
+        // NaN); the dictionary is mapped back
+        const bool mapped_key = key_needs_mapping(column.type);
         eg_column_view src = column;
-        DevBuf wide;
-        if (kb < 4) {
-            wide = DevBuf((size_t)std::max<int64_t>(column.size, 1) * 4, s);
-            if (column.size > 0) {
-                hipLaunchKernelGGL(widen_kernel, dim3((unsigned)small_grid(column.size)), dim3(256), 0, s, keys, kb,
-                                   is_signed_int(column.type) ? 1 : 0, column.size, wide.get<int32_t>());
-                EG_LAUNCH_CHECK();
-            }
-            keys = wide.get();
-            kb = 4;
-            src.data = wide.get();
+        DevBuf mapped;
+        if (mapped_key) {
+            mapped = map_keys(keys, column.type, column.size, s);
+            keys = mapped.get();
+            src.data = mapped.get();
             src.offset = 0;
-            src.type = EG_INT32;
+            src.type = mapped_key_type(column.type);
         }
         // the distinct keys: ascending (sort) for with_order, any order (hash) otherwise
         eg_groupby_hint h;
@@ -542,15 +524,10 @@ int eg_dictionary_encode(eg_column_view column, int32_t encode_type, int64_t dic
         out->reserved = 0;
         out->offset = 0;
         *out_dict_size = dv.size;
-        if (out_dict && orig_kb < kb) {
-            DevBuf narrow((size_t)std::max<int64_t>(dv.size, 1) * orig_kb, s);
-            if (dv.size > 0) {
-                hipLaunchKernelGGL(narrow_kernel, dim3((unsigned)small_grid(dv.size)), dim3(256), 0, s,
-                                   (const int32_t*)dv.data, orig_kb, dv.size, narrow.get());
-                EG_LAUNCH_CHECK();
-            }
+        if (out_dict && mapped_key) {
+            DevBuf back = unmap_keys(dv.data, dv.size, column.type, s);
             EG_HIP(hipStreamSynchronize(s));
-            out_dict->data = narrow.release();
+            out_dict->data = back.release();
             out_dict->size = dv.size;
             out_dict->type = column.type;
             out_dict->reserved = 0;

@@ -159,6 +159,10 @@ private:
     std::chrono::steady_clock::time_point t0_;
 };
 
+// Counts one occurrence of an event (a retry, say) under `name` while timing
+// is enabled; queried like a kernel, with 0 ms.
+void timing_count(const char* name);
+
 // ---- fills ------------------------------------------------------------------------
 // Zero-fill with a kernel (graph-safe, no memset nodes).
 void fill_zero(void* p, size_t bytes, hipStream_t s);
@@ -202,6 +206,9 @@ struct Result {
 
 // Allocates the output columns for `groups` rows and fills an OutSpec.
 void alloc_outputs(const Plan& plan, int64_t groups, hipStream_t s, Result& res, OutSpec& os);
+// Fills an OutSpec for output columns the caller owns (out_data: one pointer
+// per plan output, in order).
+void set_outspec(const Plan& plan, void* key_data, void* const* out_data, OutSpec& os);
 
 // ---- strategies ----------------------------------------------------------------------
 // Lanes of a wave on one group before the wave combines their rows (65 =
@@ -223,6 +230,15 @@ struct HashTuning {
 };
 
 Result run_hash(const Plan& plan, int64_t group_bound, hipStream_t s, const HashTuning& t = HashTuning{});
+
+// ---- key mapping (keynorm.hip) -------------------------------------------------------
+// Single key columns of 1- or 2-byte integers, BOOL8 or floats run on a 4- or
+// 8-byte integer column that keeps equality and order (floats: -0.0 = +0.0,
+// one NaN); group keys are mapped back.
+bool key_needs_mapping(int32_t type);
+int32_t mapped_key_type(int32_t type);
+DevBuf map_keys(const void* keys, int32_t type, int64_t n, hipStream_t s);
+DevBuf unmap_keys(const void* mapped, int64_t n, int32_t type, hipStream_t s);
 
 // ---- generator -------------------------------------------------------------------------
 void gen_keys_uniform(void* out, int32_t type, int64_t begin, int64_t count, int64_t n,

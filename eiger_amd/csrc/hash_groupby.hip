@@ -27,6 +27,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "eg_agg_kernels.hpp"
 #include "eg_atomics.hpp"
@@ -74,9 +76,12 @@ __global__ void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* c
     }
 }
 
+// use_jit: take the specialised kernel once it is loaded (loading a module
+// allocates device memory for its code, so callers that must not allocate
+// pass false and always run the generic kernel).
 template <int KB, class VL>
 void launch_agg(const DeviceInfo& di, const HashTuning& t, const KeyDesc& key, int64_t n, const AggSpec& spec,
-                const GTable& gt, const HashLayout& L, hipStream_t s)
+                const GTable& gt, const HashLayout& L, hipStream_t s, bool use_jit)
 {
     auto kern = hash_agg_kernel<KB, VL>;
     enable_dynamic_lds((const void*)kern);
@@ -88,7 +93,7 @@ void launch_agg(const DeviceInfo& di, const HashTuning& t, const KeyDesc& key, i
     };
     KernelTimer timer("hash_agg", s);
     // the kernel with this request's ops compiled in, once it is available
-    if (hipFunction_t f = jit::lookup(jit::Kind::HASH_AGG, KB, jit::vlayout_name<VL>(), spec, n)) {
+    if (hipFunction_t f = use_jit ? jit::lookup(jit::Kind::HASH_AGG, KB, jit::vlayout_name<VL>(), spec, n) : nullptr) {
         const int64_t grid = grid_for(t.blocks_per_cu > 0 ? t.blocks_per_cu : jit::occupancy(f, block, L.lds_bytes));
         KeyDesc a_key = key;
         int64_t a_n = n;
@@ -110,9 +115,9 @@ void launch_agg(const DeviceInfo& di, const HashTuning& t, const KeyDesc& key, i
 // columns on 16-byte aligned data, the generic typed-load kernel otherwise.
 template <int KB>
 void dispatch_layout(bool vec, const DeviceInfo& di, const HashTuning& t, const KeyDesc& key, int64_t n,
-                     const AggSpec& spec, const GTable& gt, const HashLayout& L, hipStream_t s)
+                     const AggSpec& spec, const GTable& gt, const HashLayout& L, hipStream_t s, bool use_jit = true)
 {
-    with_vlayout(spec, vec, [&](auto vl) { launch_agg<KB, decltype(vl)>(di, t, key, n, spec, gt, L, s); });
+    with_vlayout(spec, vec, [&](auto vl) { launch_agg<KB, decltype(vl)>(di, t, key, n, spec, gt, L, s, use_jit); });
 }
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
@@ -183,6 +188,7 @@ Result run_hash(const Plan& plan, int64_t group_bound, hipStream_t s, const Hash
             // The table filled up: the group bound was too small.  Retry with a
             // table sized for every row being its own group.
             if (slots >= 2 * n) raise(EG_ERR_CAPACITY, "hash table overflow");
+            timing_count("hash_retry");
             int64_t ns = slots;
             while (ns < 2 * n) ns <<= 1;
             slots = ns;

@@ -392,6 +392,7 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
         if (table_full || groups > capacity) {
             // the HBM table or the output ran out: retry with room for every row
             if (capacity >= n && slots >= 2 * n) raise(EG_ERR_CAPACITY, "partition group-by overflow");
+            timing_count(table_full ? "partition_retry_table" : "partition_retry_capacity");
             if (groups > capacity) capacity = n;
             if (table_full) {
                 int64_t ns = slots;

@@ -387,6 +387,13 @@ HostPhase::~HostPhase()
     g_recs.push_back(TimingRec{name_, nullptr, nullptr, ms});
 }
 
+void timing_count(const char* name)
+{
+    if (!g_timing) return;
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    g_recs.push_back(TimingRec{name, nullptr, nullptr, 0.0});
+}
+
 int timing_enable(int on)
 {
     g_timing = on != 0;

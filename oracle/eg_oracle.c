@@ -531,12 +531,31 @@ static void emit(const plan_t* p, int j, const state_t* st, void* out, int64_t g
     else store_int(out, ot, g, (int64_t)s->u);
 }
 
-static int key_type_ok(int32_t t) { return is_signed_int(t) || is_unsigned_int(t); }
+static int key_type_ok(int32_t t)
+{
+    return is_signed_int(t) || is_unsigned_int(t) || t == ORC_BOOL8 || t == ORC_FLOAT32 || t == ORC_FLOAT64;
+}
+
+/* Float keys compare as numbers with operator== (memory/row_utils.cuh:16-21:
+ * -0.0 equals +0.0), except that every NaN is one key, after +inf.  The
+ * value (a float32 widened exactly) is ranked by its IEEE-754 bits: the bit
+ * order of non-negative doubles is their numeric order, and reversed for
+ * negative ones. */
+static uint64_t float_key_ord(double v)
+{
+    if (isnan(v)) return UINT64_MAX;
+    if (v == 0.0) v = 0.0;
+    uint64_t b;
+    memcpy(&b, &v, sizeof b);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
 
 /* key comparison value: signed keys as int64, unsigned as uint64 mapped into
- * the same order by flipping the sign bit */
+ * the same order by flipping the sign bit; floats by float_key_ord */
 static uint64_t key_ord(const void* k, int32_t t, int64_t i)
 {
+    if (t == ORC_FLOAT32) return float_key_ord((double)((const float*)k)[i]);
+    if (t == ORC_FLOAT64) return float_key_ord(((const double*)k)[i]);
     if (is_signed_int(t)) return (uint64_t)as_i64(k, t, i) ^ 0x8000000000000000ULL;
     return raw_bits(k, t, i);
 }
@@ -650,6 +669,7 @@ int orc_groupby_parallel(const orc_col* key, const orc_col* vals, int32_t nvals,
 {
     plan_t p;
     if (check_inputs(key, vals, nvals)) return -1;
+    if (key->type == ORC_FLOAT32 || key->type == ORC_FLOAT64) return fail("float keys: brute-force oracle only");
     if (build_plan(vals, nvals, reqs, nreqs, agg_count, &p)) return -1;
     const int64_t n = key->size;
     const int P = 1 << PAR_PB;

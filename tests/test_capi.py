@@ -79,8 +79,8 @@ def test_multi_column_key_requests_rejected_before_device_work():
 
 
 def test_dictionary_encode_arguments_rejected_before_device_work():
-    """compress/dictionary_encode*.cu: integral key columns only; the
-    dictionary-ready form encodes to INT32 only and needs a dictionary of the
+    """compress/dictionary_encode*.cu: the dictionary-ready form takes
+    integral key columns, encodes to INT32 only and needs a dictionary of the
     column's type; the self-built form needs an integral encoded type."""
     def status(fn):
         with pytest.raises(G.GamaxError) as e:
@@ -91,7 +91,6 @@ def test_dictionary_encode_arguments_rejected_before_device_work():
     assert status(lambda: G.dictionary_encode_with_dict(_cv(), _cv(), encode_type=G.type_id.INT64)) == \
         G.EG_ERR_UNSUPPORTED
     assert status(lambda: G.dictionary_encode_with_dict(_cv(), _cv(t=G.type_id.INT64))) == G.EG_ERR_INVALID
-    assert status(lambda: G.dictionary_encode(f32)) == G.EG_ERR_UNSUPPORTED
     assert status(lambda: G.dictionary_encode(_cv(), G.type_id.FLOAT64)) == G.EG_ERR_UNSUPPORTED
     assert status(lambda: G.dictionary_encode_with_dict(_cv(ptr=0), _cv())) == G.EG_ERR_INVALID  # null data
 
@@ -103,9 +102,20 @@ def test_no_requests_is_invalid():
     assert "No aggregation" in str(e.value)
 
 
-def test_float_keys_are_unsupported():
+def test_float_keys_are_unsupported_by_partition():
+    """groupby_partition.cu:556-567: non-integral keys are rejected by the
+    partition strategy (forced or through the hint); STRING keys everywhere."""
+    for t in (G.type_id.FLOAT32, G.type_id.FLOAT64):
+        for fn in (G.partition_groupby, G.partition_groupby_gfur):
+            with pytest.raises(G.GamaxError) as e:
+                fn([_cv(t=t)], [_cv()], [(0, G.aggregation_id.SUM)])
+            assert e.value.status == G.EG_ERR_UNSUPPORTED
+        with pytest.raises(G.GamaxError) as e:
+            G.groupby([_cv(t=t)], [_cv()], [(0, G.aggregation_id.SUM)],
+                      G.groupby_hint(groupby_algo=G.groupby_algo.PARTITION))
+        assert e.value.status == G.EG_ERR_UNSUPPORTED
     with pytest.raises(G.GamaxError) as e:
-        G.groupby([_cv(t=G.type_id.FLOAT32)], [_cv()], [(0, G.aggregation_id.SUM)])
+        G.groupby([_cv(t=G.type_id.STRING)], [_cv()], [(0, G.aggregation_id.SUM)])
     assert e.value.status == G.EG_ERR_UNSUPPORTED
 
 

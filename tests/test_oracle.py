@@ -521,3 +521,42 @@ def test_take_and_sort_by_key_fast():
     fast, slow = O.sort_by_key_fast(cols), O.sort_by_key(cols)
     for a, b in zip(fast, slow):
         np.testing.assert_array_equal(a, b)
+
+
+# ---- float and narrow integer keys --------------------------------------------
+def _bruteforce_groups(keys, vals):
+    """Group-by in pure Python on a dict: float keys compare with == (so -0.0
+    and +0.0 meet: equal floats hash equally), every NaN is one group.
+    COUNT, SUM, MIN, MAX of an int64 column, groups in ascending key order
+    with the NaN group last."""
+    g = {}
+    for k, v in zip(keys.tolist(), vals.tolist()):
+        kk = "nan" if (isinstance(k, float) and k != k) else k
+        c, s, lo, hi = g.get(kk, (0, 0, v, v))
+        g[kk] = (c + 1, s + v, min(lo, v), max(hi, v))
+    order = sorted((k for k in g if k != "nan")) + (["nan"] if "nan" in g else [])
+    return order, [g[k] for k in order]
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64, np.int8, np.int16, np.uint8, np.uint16])
+def test_oracle_float_and_narrow_keys_match_python_bruteforce(dt):
+    rng = np.random.default_rng(5)
+    if np.dtype(dt).kind == "f":
+        pool = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 1.5, -1.5, 1e-30, -2.0, 3.25,
+                         np.finfo(dt).max, np.finfo(dt).min, np.finfo(dt).tiny], dtype=dt)
+        pool = np.concatenate([pool, rng.standard_normal(40).astype(dt)])
+    else:
+        info = np.iinfo(dt)
+        pool = np.concatenate([[info.min, info.max, 0], rng.integers(info.min, info.max, 30, endpoint=True)]).astype(dt)
+    keys = pool[rng.integers(0, len(pool), 3000)]
+    vals = rng.integers(-(1 << 40), 1 << 40, len(keys)).astype(np.int64)
+    reqs = [(-1, O.COUNT_ALL), (0, O.SUM), (0, O.MIN), (0, O.MAX)]
+    got = O.groupby(keys, [vals], reqs)
+    order, states = _bruteforce_groups(keys, vals)
+    assert len(got[0]) == len(order)
+    for i, (k, (c, s, lo, hi)) in enumerate(zip(order, states)):
+        if k == "nan":
+            assert np.isnan(got[0][i])
+        else:
+            assert got[0][i] == k  # numeric equality: -0.0 == 0.0
+        assert (got[1][i], got[2][i], got[3][i], got[4][i]) == (c, s, lo, hi)
