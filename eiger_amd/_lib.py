@@ -90,6 +90,7 @@ def _proto(L):
     for name in ("eg_groupby", "eg_groupby_hash", "eg_groupby_sort", "eg_groupby_sort_gfur",
                  "eg_groupby_partition", "eg_groupby_partition_gfur"):
         f(name, C.c_int, gb)
+    f("eg_groupby_hash_noalloc", C.c_int, [PCV, i64, PCV, i64, PAR, i64, i64, PCV, i64, C.POINTER(i64), vp])
     f("eg_distinct_hash", C.c_int, [PCV, i64, PH, vp, PT])
     f("eg_distinct_sort", C.c_int, [PCV, i64, PH, vp, PT])
     f("eg_aggregate", C.c_int, [PCV, i64, PAR, i64, vp, PT])

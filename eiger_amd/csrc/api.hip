@@ -533,6 +533,52 @@ int eg_groupby_partition_gfur(const eg_column_view* keys, int64_t num_keys,
     return groupby_entry(Entry::PARTITION_GFUR, keys, num_keys, vals, num_vals, reqs, num_reqs, hint, stream, out);
 }
 
+int eg_groupby_hash_noalloc(const eg_column_view* keys, int64_t num_keys, const eg_column_view* vals,
+                            int64_t num_vals, const eg_agg_request* reqs, int64_t num_reqs,
+                            int64_t groups_upper_bound, const eg_column_view* out_cols, int64_t num_out_cols,
+                            int64_t* num_groups, void* stream)
+{
+    return guard([&] {
+        if (!num_groups) raise(EG_ERR_INVALID, "num_groups is null");
+        *num_groups = 0;
+        if (num_keys != 1 || !keys)
+            raise(num_keys > 1 ? EG_ERR_UNSUPPORTED : EG_ERR_INVALID,
+                  "the no-allocation group-by takes exactly one key column");
+        if (num_vals < 0 || (num_vals > 0 && !vals)) raise(EG_ERR_INVALID, "bad value columns");
+        ViewIn k = view_in(keys[0]);
+        std::vector<ViewIn> vs;
+        for (int64_t i = 0; i < num_vals; ++i) vs.push_back(view_in(vals[i]));
+        Plan plan;
+        try {
+            plan = make_plan(&k, vs, reqs, num_reqs, false, false);
+        } catch (const WideError& e) {
+            raise(EG_ERR_UNSUPPORTED, std::string("no-allocation group-by: ") + e.what());
+        }
+        // caller-owned outputs: [key, one column per request], each typed as the result
+        if (!out_cols || num_out_cols != 1 + (int64_t)plan.outs.size())
+            raise(EG_ERR_INVALID, "expected " + std::to_string(1 + plan.outs.size()) + " output columns");
+        int64_t capacity = INT64_MAX;
+        void* outs[kMaxOuts] = {};
+        void* key_out = nullptr;
+        for (int64_t j = 0; j < num_out_cols; ++j) {
+            const eg_column_view& o = out_cols[j];
+            const int32_t want = j == 0 ? plan.key_type : plan.outs[(size_t)j - 1].type;
+            if (o.type != want) raise(EG_ERR_INVALID, "output column " + std::to_string(j) + " has the wrong type");
+            if (o.size < 0 || o.offset < 0) raise(EG_ERR_INVALID, "bad output column view");
+            if (o.size > 0 && !o.data) raise(EG_ERR_INVALID, "null output column data");
+            capacity = std::min(capacity, o.size);
+            void* p = o.data ? (char*)o.data + o.offset * type_size(o.type) : nullptr;
+            if (j == 0) key_out = p;
+            else outs[j - 1] = p;
+        }
+        if (plan.rows == 0) return;
+        const int64_t bound = groups_upper_bound > 0 ? std::min(groups_upper_bound, plan.rows) : plan.rows;
+        OutSpec os;
+        set_outspec(plan, key_out, outs, os);
+        *num_groups = run_hash_noalloc(plan, bound, os, capacity, as_stream(stream));
+    });
+}
+
 // Distinct key rows: the group-by with one COUNT_ALL request, count dropped.
 static int distinct_entry(bool sort, const eg_column_view* keys, int64_t num_keys, eg_groupby_hint* hint,
                           void* stream, eg_table* out)

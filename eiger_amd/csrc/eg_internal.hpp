@@ -231,6 +231,15 @@ struct HashTuning {
 
 Result run_hash(const Plan& plan, int64_t group_bound, hipStream_t s, const HashTuning& t = HashTuning{});
 
+// The hash strategy into caller-owned outputs (os, `capacity` rows each) with
+// its HBM table in the library's static device scratch: no device memory is
+// allocated or freed.  Returns the group count; raises EG_ERR_CAPACITY when
+// the table would exceed the scratch, when the groups overflow the table the
+// scratch holds, or when they exceed `capacity` (outputs then untouched).
+// Calls that overlap on one device share the scratch and race.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+int64_t run_hash_noalloc(const Plan& plan, int64_t group_bound, const OutSpec& os, int64_t capacity, hipStream_t s);
+
 // ---- key mapping (keynorm.hip) -------------------------------------------------------
 // Single key columns of 1- or 2-byte integers, BOOL8 or floats run on a 4- or
 // 8-byte integer column that keeps equality and order (floats: -0.0 = +0.0,

@@ -208,6 +208,100 @@ Result run_hash(const Plan& plan, int64_t group_bound, hipStream_t s, const Hash
     }
 }
 
+// ---- no-allocation variant ---------------------------------------------------------
+// The HBM table of run_hash_noalloc lives in this array: one copy per device,
+// loaded with the module and held until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it only stands for the
+// symbol); it uses the device address that hipGetSymbolAddress returns for
+// the current device, looked up once per device.
+namespace {
+__device__ __align__(256) unsigned long long scratch[kNoAllocScratchBytes / 8];
+
+unsigned long long* scratch_ptr()
+{
+    static std::mutex mu;
+    static std::vector<void*> ptrs;
+    int dev = 0;
+    EG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if ((size_t)dev >= ptrs.size()) ptrs.resize((size_t)dev + 1, nullptr);
+    if (!ptrs[(size_t)dev]) {
+        void* p = nullptr;
+        EG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) raise(EG_ERR_DEVICE, "no device address for the group-by scratch");
+        ptrs[(size_t)dev] = p;
+    }
+    return static_cast<unsigned long long*>(ptrs[(size_t)dev]);
+}
+}  // namespace
+
+int64_t run_hash_noalloc(const Plan& plan, int64_t group_bound, const OutSpec& os, int64_t capacity, hipStream_t s)
+{
+    const DeviceInfo& di = device_info();
+    const int64_t n = plan.rows;
+    const int nops = plan.spec.nops;
+    const int W = table_words_per_slot(nops);
+    const int KB = plan.key.bytes;
+    const HashTuning t;
+    const HashLayout L = plan_hash_layout(nops, KB, group_bound, t);
+    bool vec = aligned16(plan.key.data);
+    for (int c = 0; c < plan.spec.nvcols; ++c) vec = vec && aligned16(plan.spec.cols[c].data);
+
+    // table: slots * W words, then the side slot (W words) and 4 meta words;
+    // meta[3] is the compaction's output counter
+    const size_t max_words = kNoAllocScratchBytes / 8;
+    auto words_of = [&](int64_t slots) { return (size_t)slots * (size_t)W + (size_t)W + 4; };
+    const int64_t bound = std::max<int64_t>(1, std::min<int64_t>(group_bound, n));
+    int64_t slots = 1024;
+    while (slots < 2 * bound) slots <<= 1;
+    if (words_of(slots) > max_words)
+        raise(EG_ERR_CAPACITY, "groups_upper_bound " + std::to_string(group_bound) + " needs a table of " +
+                                   std::to_string(words_of(slots) * 8) + " bytes, above the " +
+                                   std::to_string(kNoAllocScratchBytes) + "-byte no-allocation scratch");
+    int64_t max_slots = slots;
+    while (words_of(max_slots * 2) <= max_words) max_slots <<= 1;
+
+    unsigned long long* base = scratch_ptr();
+    for (;;) {
+        GTable gt;
+        gt.base = base;
+        gt.side = base + (size_t)slots * W;
+        gt.meta = gt.side + W;
+        gt.mask = (uint64_t)slots - 1;
+        gt.W = W;
+        fill_zero(base, words_of(slots) * 8, s);  // a kernel: every call starts from an empty table
+        if (KB == 4) dispatch_layout<4>(vec, di, t, plan.key, n, plan.spec, gt, L, s, false);
+        else dispatch_layout<8>(vec, di, t, plan.key, n, plan.spec, gt, L, s, false);
+        int64_t meta[3] = {0, 0, 0};
+        EG_HIP(hipMemcpyAsync(meta, gt.meta, sizeof meta, hipMemcpyDeviceToHost, s));
+        EG_HIP(hipStreamSynchronize(s));
+        if (meta[2] != 0) {
+            // more groups than the bound said: once more with the largest
+            // table the scratch holds
+            if (slots < max_slots) {
+                slots = max_slots;
+                continue;
+            }
+            raise(EG_ERR_CAPACITY, "more groups than the no-allocation scratch's table holds (" +
+                                       std::to_string(slots / 2) + " at load 1/2)");
+        }
+        const int64_t groups = meta[0] + (meta[1] != 0 ? 1 : 0);
+        if (groups > capacity)
+            raise(EG_ERR_CAPACITY, std::to_string(groups) + " groups exceed the output capacity of " +
+                                       std::to_string(capacity) + " rows");
+        if (groups > 0) {
+            const uint64_t total = gt.mask + 2;
+            const int64_t grid =
+                std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + 255) / 256), (int64_t)di.num_cus * 8));
+            KernelTimer timer("hash_compact", s);
+            hipLaunchKernelGGL(hash_compact_kernel, dim3((unsigned)grid), dim3(256), 0, s, gt, os, gt.meta + 3);
+            EG_LAUNCH_CHECK();
+        }
+        EG_HIP(hipStreamSynchronize(s));
+        return groups;
+    }
+}
+
 // ---- HbmTable ---------------------------------------------------------------------
 int table_words_per_slot(int nops)
 {

@@ -314,6 +314,19 @@ def hash_groupby(key, val, agg_requests, hint=None, stream=None) -> table:
     return _groupby_call("eg_groupby_hash", key, val, agg_requests, hint, stream)
 
 
+def hash_groupby_noalloc(key, val, agg_requests, out_columns, groups_upper_bound: int = 0, stream=None) -> int:
+    """The hash strategy into caller-owned columns with nothing allocated
+    (eg_groupby_hash_noalloc; no counterpart in the reference).  out_columns:
+    [key, one per request] views typed as hash_groupby's result, each view's
+    size its capacity in rows.  Returns the number of groups written."""
+    keys, vals, outs = _tv(key), _tv(val), _tv(out_columns)
+    ng = C.c_int64(0)
+    _check(lib().eg_groupby_hash_noalloc(_views(keys), len(keys), _views(vals), len(vals), _reqs(agg_requests),
+                                         len(agg_requests), int(groups_upper_bound), _views(outs), len(outs),
+                                         C.byref(ng), _stream(stream)))
+    return int(ng.value)
+
+
 def sort_groupby(key, val, agg_requests, hint=None, stream=None) -> table:
     """gamax::groupby::sort::groupby (groupby_sort.hpp:17-22)."""
     return _groupby_call("eg_groupby_sort", key, val, agg_requests, hint, stream)

@@ -151,6 +151,29 @@ int eg_groupby_partition_gfur(const eg_column_view* keys, int64_t num_keys,
                               const eg_agg_request* reqs, int64_t num_reqs,
                               eg_groupby_hint* hint, void* stream, eg_table* out);
 
+/* The hash strategy with nothing allocated: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * requests and results as eg_groupby_hash with one key column, written into
+ * caller-owned columns out_cols = [key, one per request], typed as that
+ * result; each column's size is its capacity in rows (offset honoured).
+ * *num_groups receives the group count; synchronous at return.  The HBM
+ * table lives in a static device array of 32 MiB per GPU (held until the
+ * process ends), so groups_upper_bound (<= 0: the row count) is limited to
+ * what that table holds at load 1/2: 2^20 / W groups, W the power of two
+ * >= 1 + aggregation states (262144 groups for up to 3 states, 131072 for
+ * 4 to 7).  A larger
+ * bound, more groups than the table holds, or more groups than the output
+ * capacity give EG_ERR_CAPACITY (outputs untouched).  Calls that overlap on
+ * one GPU share that array and race: serialise them per device.  The
+ * generic kernels run (no run-time specialisation). */
+int eg_groupby_hash_noalloc(const eg_column_view* keys, int64_t num_keys,
+                            const eg_column_view* vals, int64_t num_vals,
+                            const eg_agg_request* reqs, int64_t num_reqs,
+                            int64_t groups_upper_bound,
+                            const eg_column_view* out_cols, int64_t num_out_cols,
+                            int64_t* num_groups, void* stream);
+
 /* Distinct rows of the key columns, one per unique key (or key tuple):
  *   hash::distinct  operators/groupby/groupby_hash.hpp:24-32 (any key
  *                   shape the group-by takes; first-encounter order is not
