@@ -419,7 +419,16 @@ def dispatch_eval(G, nq, seed=3, key_seed0=3_000_000):
     best = t[:, :3].min(axis=1)
     arg = t[:, :3].argmin(axis=1)
     tot = t.sum(axis=0)
-    return {"queries": nq, "query_seed": seed, "best_ms": round(float(best.sum()), 1),
+    rows = np.array([sp[1] for sp in specs], dtype=np.int64)
+    by_rows = {}
+    for lo, hi, name in ((0, 1 << 17, "rows<2^17"), (1 << 17, 1 << 22, "2^17<=rows<2^22"), (1 << 22, 1 << 62, "rows>=2^22")):
+        sel = (rows >= lo) & (rows < hi)
+        if sel.any():
+            by_rows[name] = {"queries": int(sel.sum()), "best_ms": round(float(best[sel].sum()), 2),
+                             "mlp_ms": round(float(t[sel, 3].sum()), 2), "rule_ms": round(float(t[sel, 4].sum()), 2),
+                             "fixed_ms": {nm: round(float(t[sel, j].sum()), 2) for j, nm in enumerate(names)},
+                             "mlp_picks_best": round(float((picks[sel, 0] == arg[sel]).mean()), 4)}
+    return {"queries": nq, "query_seed": seed, "best_ms": round(float(best.sum()), 1), "by_rows": by_rows,
             "mlp_ms": round(float(tot[3]), 1), "rule_ms": round(float(tot[4]), 1),
             "fixed_ms": {nm: round(float(tot[j]), 1) for j, nm in enumerate(names)},
             "mlp_over_best": round(float(tot[3] / best.sum()), 4),

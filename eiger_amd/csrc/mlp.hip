@@ -499,6 +499,12 @@ LoadedModel& loaded_model()
     return m;
 }
 
+// Inputs below this many rows go to hash without sampling: the call's fixed
+// costs dominate there (hash: one aggregation launch; sort and partition:
+// three launches per radix pass), and the sample's features and inference
+// (~0.08 ms, one synchronisation) would cost more than any choice saves.
+constexpr int64_t kSmallRows = (int64_t)1 << 17;
+
 // Strategy from the model's outputs o[0..3] (class scores of hash / sort /
 // partition, log2 group count / 32).  A large input whose estimated group
 // count fits one LDS table goes to hash: one streaming pass, where every other
@@ -506,6 +512,7 @@ LoadedModel& loaded_model()
 // the fastest for all such queries: rows >= 2^24, estimate <= 2048).
 int algo_of(const float* o, int64_t rows)
 {
+    if (rows < kSmallRows) return EG_ALGO_HASH;
     const double card = std::exp2(std::min(62.0, std::max(0.0, (double)o[3] * 32.0)));
     if (rows >= ((int64_t)1 << 24) && card <= 2048.0) return EG_ALGO_HASH;
     const int best = (o[1] > o[0] && o[1] >= o[2]) ? 1 : (o[2] > o[0] && o[2] > o[1]) ? 2 : 0;
@@ -519,6 +526,13 @@ bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_
     LoadedModel& m = loaded_model();
     std::lock_guard<std::mutex> lk(m.mu);
     if (!m.loaded) return false;
+    if (rows < kSmallRows) {
+        // hash, sized by the caller's bound (or the row count)
+        *algo = EG_ALGO_HASH;
+        if (card_est) *card_est = rows;
+        if (singletons) *singletons = 0.f;
+        return true;
+    }
     FQuery q{keys, rows, group_bound, type_size(key_type), nops, value_bytes, 0};
     {
         KernelTimer timer("mlp_select", s);
@@ -734,7 +748,9 @@ int eg_mlp_select_batch(const eg_mlp_query* queries, int64_t num, int32_t* algos
             algos[i] = eg::algo_of(o, queries[i].rows);
             if (card_est) {
                 const double lg = std::min(62.0, std::max(0.0, (double)o[3] * 32.0));
-                card_est[i] = (int64_t)std::min<double>((double)std::max<int64_t>(queries[i].rows, 1), std::exp2(lg));
+                card_est[i] = queries[i].rows < eg::kSmallRows  // as mlp_select: hash sized by the row count
+                                  ? queries[i].rows
+                                  : (int64_t)std::min<double>((double)std::max<int64_t>(queries[i].rows, 1), std::exp2(lg));
             }
         }
     });

@@ -20,24 +20,67 @@ namespace {
 constexpr int kP = 14;
 constexpr int kM = 1 << kP;
 
-__global__ __launch_bounds__(256) void hll_kernel(const void* data, int bytes, int64_t n,
-                                                  uint8_t* __restrict__ partial)
+// One register update: the register is read first, and the LDS atomic is
+// issued only when it would raise it (after the first rows most do not;
+// registers only grow, so a stale read never skips a needed update).
+__device__ __forceinline__ void hll_add(uint32_t* reg, uint64_t raw)
+{
+    const uint64_t h = dev::fmix64(raw ^ 0x9e3779b97f4a7c15ull);
+    const uint32_t idx = (uint32_t)(h >> (64 - kP));
+    const uint64_t w = (h << kP) | (1ull << (kP - 1));
+    const uint32_t rho = (uint32_t)__clzll((long long)w) + 1u;
+    if (__hip_atomic_load(&reg[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < rho) atomicMax(&reg[idx], rho);
+}
+
+// KB = 4 or 8: 16-byte vector loads when the column is 16-byte aligned;
+// KB = 0: any width, one typed load per row.
+template <int KB>
+__global__ __launch_bounds__(1024) void hll_kernel(const void* data, int bytes, int64_t n,
+                                                   uint8_t* __restrict__ partial)
 {
     __shared__ uint32_t reg[kM];
     for (int i = threadIdx.x; i < kM; i += blockDim.x) reg[i] = 0u;
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint64_t raw = dev::load_raw(data, bytes, i);
-        uint64_t h = dev::fmix64(raw ^ 0x9e3779b97f4a7c15ull);
-        uint32_t idx = (uint32_t)(h >> (64 - kP));
-        uint64_t w = (h << kP) | (1ull << (kP - 1));
-        uint32_t rho = (uint32_t)__clzll((long long)w) + 1u;
-        atomicMax(&reg[idx], rho);
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t done = 0;
+    if constexpr (KB == 4 || KB == 8) {
+        constexpr int kPer = 16 / KB;
+        const int64_t nvec = (((uintptr_t)data & 15u) == 0u) ? n / kPer : 0;
+        const uint4* v = static_cast<const uint4*>(data);
+        for (int64_t i = gid; i < nvec; i += stride) {
+            const uint4 x = v[i];
+            if constexpr (KB == 4) {
+                hll_add(reg, x.x);
+                hll_add(reg, x.y);
+                hll_add(reg, x.z);
+                hll_add(reg, x.w);
+            } else {
+                hll_add(reg, (uint64_t)x.x | ((uint64_t)x.y << 32));
+                hll_add(reg, (uint64_t)x.z | ((uint64_t)x.w << 32));
+            }
+        }
+        done = nvec * kPer;
     }
+    for (int64_t i = done + gid; i < n; i += stride) hll_add(reg, dev::load_raw(data, bytes, i));
     __syncthreads();
     uint8_t* out = partial + (size_t)blockIdx.x * kM;
     for (int i = threadIdx.x; i < kM; i += blockDim.x) out[i] = (uint8_t)reg[i];
+}
+
+void launch_hll(const void* data, int bytes, int64_t n, int parts, uint8_t* partial, hipStream_t s)
+{
+    if (bytes == 4) hipLaunchKernelGGL(hll_kernel<4>, dim3(parts), dim3(1024), 0, s, data, bytes, n, partial);
+    else if (bytes == 8) hipLaunchKernelGGL(hll_kernel<8>, dim3(parts), dim3(1024), 0, s, data, bytes, n, partial);
+    else hipLaunchKernelGGL(hll_kernel<0>, dim3(parts), dim3(1024), 0, s, data, bytes, n, partial);
+    EG_LAUNCH_CHECK();
+}
+
+// Workgroups of the register pass: two per CU (64 KB of registers each),
+// fewer for short columns.
+int hll_parts(int64_t n)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + 16383) / 16384, (int64_t)device_info().num_cus * 2));
 }
 
 __global__ void hll_merge_kernel(const uint8_t* __restrict__ partial, int parts, uint8_t* __restrict__ out)
@@ -69,12 +112,10 @@ double hll_estimate(const uint8_t* reg)
 // Registers of one column into regs (kM bytes, device); asynchronous.
 void hll_registers(const void* data, int bytes, int64_t n, uint8_t* regs, DevBuf& partial, hipStream_t s)
 {
-    const DeviceInfo& di = device_info();
-    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>((n + 4095) / 4096, (int64_t)di.num_cus * 2));
+    const int parts = hll_parts(n);
     partial = DevBuf((size_t)parts * kM, s);
     KernelTimer timer("hll", s);
-    hipLaunchKernelGGL(hll_kernel, dim3(parts), dim3(256), 0, s, data, bytes, n, partial.get<uint8_t>());
-    EG_LAUNCH_CHECK();
+    launch_hll(data, bytes, n, parts, partial.get<uint8_t>(), s);
     hipLaunchKernelGGL(hll_merge_kernel, dim3(kM / 256), dim3(256), 0, s, (const uint8_t*)partial.get<uint8_t>(), parts,
                        regs);
     EG_LAUNCH_CHECK();
@@ -86,14 +127,12 @@ int64_t ndv_approx(const void* data, int32_t type, int64_t n, hipStream_t s)
     const int bytes = type_size(type);
     if (!bytes) raise(EG_ERR_UNSUPPORTED, "compute_ndv_approx: unsupported column type");
     if (n <= 0) return 0;
-    const DeviceInfo& di = device_info();
-    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>((n + 4095) / 4096, (int64_t)di.num_cus * 2));
+    const int parts = hll_parts(n);
     DevBuf partial((size_t)parts * kM + kM, s);
     uint8_t* pp = partial.get<uint8_t>();
     {
         KernelTimer timer("hll", s);
-        hipLaunchKernelGGL(hll_kernel, dim3(parts), dim3(256), 0, s, data, bytes, n, pp);
-        EG_LAUNCH_CHECK();
+        launch_hll(data, bytes, n, parts, pp, s);
         hipLaunchKernelGGL(hll_merge_kernel, dim3(kM / 256), dim3(256), 0, s, pp, parts, pp + (size_t)parts * kM);
         EG_LAUNCH_CHECK();
     }
