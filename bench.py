@@ -337,8 +337,8 @@ def batched_workload(G, W, nq=4096):
     total = mb.total
     log(f"batched workload: {nq} queries, {total} rows generated in {time.time() - t0:.1f}s")
     reqs = [(0, A.SUM), (-1, A.COUNT_ALL)]
-    cands = [G.batch_tuning(tile_rows_log2=t, lds_slots_log2=c, lds_max_bound_log2=b, blocks_per_cu=w)
-             for t in (15, 17) for c in (10, 11) for b in (10, 12) for w in (1, 2)]
+    cands = [G.batch_tuning(tile_rows_log2=17, lds_slots_log2=c, lds_max_bound_log2=b, blocks_per_cu=w, radix_bits=r)
+             for c in (10, 11) for b in (10, 12) for w in (1, 2) for r in (0, 10, 12)]
     best, times = G.batch_autotune(mb.queries, T.INT32, T.INT64, reqs, cands, reps=1)
     plan = G.batch_plan(mb.queries, T.INT32, T.INT64, reqs, tuning=best)
     plan.graph_launch()
@@ -365,7 +365,8 @@ def batched_workload(G, W, nq=4096):
             "groups": int(ng.sum()),
             "algorithmic_hbm_frac": round((12 * total + 20 * int(ng.sum())) / batch_s / PEAK_HBM, 4),
             "tuning": {"tile_rows_log2": bt.tile_rows_log2, "lds_slots_log2": bt.lds_slots_log2,
-                       "lds_max_bound_log2": bt.lds_max_bound_log2, "blocks_per_cu": bt.blocks_per_cu},
+                       "lds_max_bound_log2": bt.lds_max_bound_log2, "blocks_per_cu": bt.blocks_per_cu,
+                       "radix_bits": bt.radix_bits},
             "autotune_candidates": len(cands), "autotune_ms": [round(float(x), 3) for x in times]}
 
 

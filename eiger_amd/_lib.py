@@ -47,7 +47,7 @@ class BatchQuery(C.Structure):
 class BatchTuning(C.Structure):
     """eg_batch_tuning: execution parameters of a batched group-by."""
     _fields_ = [("tile_rows_log2", C.c_int32), ("lds_slots_log2", C.c_int32),
-                ("lds_max_bound_log2", C.c_int32), ("blocks_per_cu", C.c_int32)]
+                ("lds_max_bound_log2", C.c_int32), ("blocks_per_cu", C.c_int32), ("radix_bits", C.c_int32)]
 
 
 MLP_PARAMS = 5508

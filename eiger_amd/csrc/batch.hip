@@ -19,6 +19,22 @@
 //                        query's groups into the caller's output columns at
 //                        wave-claimed positions, counting groups per query.
 //
+// Queries whose group bound is above the LDS bound are partitioned first
+// (tuning.radix_bits > 0), with no host synchronisation in between:
+//
+//   bpart_hist_kernel    per 2^14-row tile of such a query, an LDS histogram of
+//                        the low hash bits of its keys (the partition id);
+//   bpart_scan_kernel    one workgroup per query scans its [partition][tile]
+//                        counts into each tile's first row per partition;
+//   bpart_scatter_kernel moves every row (key and value) to its partition in
+//                        a scratch buffer;
+//   batch_agg_kernel     then takes one work item per partition, aggregates it
+//                        in the workgroup's LDS table (a partition holds about
+//                        half a table of groups) and writes its groups straight
+//                        to the query's output; keys the table cannot hold
+//                        spill to a small HBM table, appended by the
+//                        compaction.
+//
 // Per-query failures cannot be retried asynchronously: a table that fills
 // (group_bound below the true group count) or an output that exceeds its
 // capacity sets the query's status bit, read back by eg_batch_results.
@@ -58,15 +74,29 @@ struct BQuery {
     void* out_keys;
     void* out_aggs[EG_BATCH_MAX_AGGS];
     int32_t use_lds;
-    int32_t pad;
+    int32_t pbits;        // partition bits (0: not partitioned)
+    int64_t ntiles;       // tiles of the partition pass
+    int64_t hist_off;     // its [partition][tile] counts / row offsets
+    int64_t scratch_off;  // its rows in the partitioned scratch columns
 };
 
+// A work item: rows [begin, end) of query q (kind 0), or partition `begin`
+// of a partitioned query (kind 1).
 struct BItem {
     int32_t q;
-    int32_t pad;
+    int32_t kind;
     int64_t begin;
     int64_t end;
 };
+
+// A tile of the partition pass: rows [t * kPTile, (t + 1) * kPTile) of query q.
+struct BPTile {
+    int32_t q;
+    int32_t t;
+};
+constexpr int64_t kPTile = (int64_t)1 << 14;
+constexpr int kMaxPartBits = 12;
+constexpr int kPB = 256;  // threads of the partition-pass kernels
 
 struct BCItem {
     int32_t q;
@@ -112,15 +142,147 @@ __global__ void batch_fill_kernel(unsigned long long* p, uint64_t words)
     }
 }
 
+template <int KB>
+__device__ __forceinline__ uint32_t part_of(uint64_t raw, uint32_t mask)
+{
+    return (uint32_t)dev::bit_mix<KB>(raw) & mask;
+}
+
+template <int KB>
+__global__ __launch_bounds__(kPB) void bpart_hist_kernel(const BQuery* __restrict__ qs, const BPTile* __restrict__ tiles,
+                                                         uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t h[1 << kMaxPartBits];
+    const BPTile tt = tiles[blockIdx.x];
+    const BQuery& q = qs[tt.q];
+    const uint32_t P = 1u << q.pbits, mask = P - 1u;
+    for (uint32_t i = threadIdx.x; i < P; i += kPB) h[i] = 0u;
+    __syncthreads();
+    const int64_t b = (int64_t)tt.t * kPTile, e = min(q.rows, b + kPTile);
+    for (int64_t r = b + threadIdx.x; r < e; r += kPB) atomicAdd(&h[part_of<KB>(dev::load_raw(q.keys, KB, r), mask)], 1u);
+    __syncthreads();
+    uint32_t* out = hist + q.hist_off + tt.t;
+    for (uint32_t d = threadIdx.x; d < P; d += kPB) out[(int64_t)d * q.ntiles] = h[d];
+}
+
+// Exclusive scan of one partitioned query's counts in [partition][tile]
+// order (in place): each entry becomes the query-local row where that tile's
+// rows of that partition start.  One 1024-thread workgroup per query, 8
+// entries per thread per round.
+__global__ __launch_bounds__(1024) void bpart_scan_kernel(const BQuery* __restrict__ qs, const int32_t* __restrict__ pqs,
+                                                          uint32_t* __restrict__ hist)
+{
+    constexpr int K = 8;
+    __shared__ uint32_t wsum[16];
+    const BQuery& q = qs[pqs[blockIdx.x]];
+    const int64_t n = ((int64_t)1 << q.pbits) * q.ntiles;
+    uint32_t* a = hist + q.hist_off;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < n; base += (int64_t)K * blockDim.x) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * K;
+        uint32_t v[K], tot = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            v[k] = i0 + k < n ? a[i0 + k] : 0u;
+            tot += v[k];
+        }
+        uint32_t x = tot;  // inclusive scan over the wave
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        uint32_t before = carry, round = 0;
+        for (int i = 0; i < 16; ++i) {
+            if (i < w) before += wsum[i];
+            round += wsum[i];
+        }
+        uint32_t ex = before + x - tot;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (i0 + k < n) a[i0 + k] = ex;
+            ex += v[k];
+        }
+        carry += round;
+        __syncthreads();  // wsum is reused
+    }
+}
+
+// Moves every row of a tile to its partition in the scratch columns (the
+// order inside a partition is the order the LDS cursors hand out).
+template <int KB>
+__global__ __launch_bounds__(kPB) void bpart_scatter_kernel(const BQuery* __restrict__ qs, const BPTile* __restrict__ tiles,
+                                                            const uint32_t* __restrict__ offs, void* __restrict__ skeys,
+                                                            void* __restrict__ svals, int vbytes)
+{
+    __shared__ uint32_t cur[1 << kMaxPartBits];
+    const BPTile tt = tiles[blockIdx.x];
+    const BQuery& q = qs[tt.q];
+    const uint32_t P = 1u << q.pbits, mask = P - 1u;
+    const uint32_t* o = offs + q.hist_off + tt.t;
+    for (uint32_t d = threadIdx.x; d < P; d += kPB) cur[d] = o[(int64_t)d * q.ntiles];
+    __syncthreads();
+    const int64_t b = (int64_t)tt.t * kPTile, e = min(q.rows, b + kPTile);
+    for (int64_t r = b + threadIdx.x; r < e; r += kPB) {
+        const uint64_t raw = dev::load_raw(q.keys, KB, r);
+        const int64_t dst = q.scratch_off + atomicAdd(&cur[part_of<KB>(raw, mask)], 1u);
+        dev::store_bytes(skeys, KB, dst, raw);
+        if (svals) dev::store_bytes(svals, vbytes, dst, dev::load_raw(q.vals, vbytes, r));
+    }
+}
+
+// Group g of query q from its states: op a at st[a * stride].
+__device__ __forceinline__ void emit_batch(const BOut& bo, const BQuery& q, int64_t g, uint64_t key_enc,
+                                           const unsigned long long* st, int64_t stride)
+{
+    dev::store_bytes(q.out_keys, bo.key_bytes, g, dev::dec64(key_enc));
+#pragma unroll
+    for (int j = 0; j < EG_BATCH_MAX_AGGS; ++j) {
+        if (j >= bo.nouts) break;
+        void* d = q.out_aggs[j];
+        const uint64_t s = st[(int64_t)bo.op[j] * stride];
+        switch (bo.code[j]) {
+        case OUT_U64: dev::store_bytes(d, 8, g, s); break;
+        case OUT_F32: ((float*)d)[g] = (float)__longlong_as_double((long long)s); break;
+        case OUT_F64: ((double*)d)[g] = __longlong_as_double((long long)s); break;
+        case OUT_MAX_I: dev::store_bytes(d, bo.bytes[j], g, s ^ dev::kSign64); break;
+        case OUT_MIN_I: dev::store_bytes(d, bo.bytes[j], g, (~s) ^ dev::kSign64); break;
+        case OUT_MAX_U: dev::store_bytes(d, bo.bytes[j], g, s); break;
+        case OUT_MIN_U: dev::store_bytes(d, bo.bytes[j], g, ~s); break;
+        case OUT_MAX_F: dev::store_double_as(d, bo.bytes[j], g, dev::unord_f64(s)); break;
+        case OUT_MIN_F: dev::store_double_as(d, bo.bytes[j], g, dev::unord_f64(~s)); break;
+        case OUT_ANY:
+            if (bo.is_float[j] && bo.bytes[j] == 4) ((float*)d)[g] = (float)__longlong_as_double((long long)s);
+            else dev::store_bytes(d, bo.bytes[j], g, s);
+            break;
+        default: {
+            const double cnt = (double)(uint64_t)st[(int64_t)bo.op2[j] * stride];
+            const double sum = (bo.code[j] == OUT_AVG_I) ? (double)(int64_t)s
+                             : (bo.code[j] == OUT_AVG_U) ? (double)s
+                             : __longlong_as_double((long long)s);
+            ((double*)d)[g] = sum / cnt;
+            break;
+        }
+        }
+    }
+}
+
 template <int KB, int NV>
 __global__ __launch_bounds__(kBatchBlock) void batch_agg_kernel(const BQuery* __restrict__ qs,
                                                                 const BItem* __restrict__ items, int n_items,
                                                                 unsigned int* next_item, unsigned long long* metas,
                                                                 AggSpec spec, int key_signed, int vbytes, int W,
-                                                                int cr_log2, int probe_limit, int peel_min)
+                                                                int cr_log2, int probe_limit, int peel_min,
+                                                                const uint32_t* __restrict__ offs, const void* skeys,
+                                                                const void* svals, unsigned long long* counts,
+                                                                uint32_t* qstatus, BOut bo)
 {
     extern __shared__ __align__(16) unsigned long long smem[];
     __shared__ int s_item;
+    __shared__ unsigned long long sh[17];
     const int nops = spec.nops;
     const int lane = dev::lane_id();
     LdsTable<KB, true> tab(smem, nops, cr_log2, 0);
@@ -138,6 +300,72 @@ __global__ __launch_bounds__(kBatchBlock) void batch_agg_kernel(const BQuery* __
         const BItem w = items[it];
         const BQuery q = qs[w.q];
         const GTable gt = query_table(q, W, metas + (size_t)w.q * kMetaWords);
+        if (w.kind == 1) {
+            // One partition of a partitioned query: its rows from the scratch
+            // columns into the LDS table, its groups straight to the output.
+            const uint32_t d = (uint32_t)w.begin;
+            const uint32_t* o = offs + q.hist_off;
+            const int64_t pb = q.scratch_off + (int64_t)o[(int64_t)d * q.ntiles];
+            const int64_t pe = q.scratch_off + (d + 1 < (1u << q.pbits) ? (int64_t)o[(int64_t)(d + 1) * q.ntiles] : q.rows);
+            key.data = skeys;
+            for (int i = threadIdx.x; i < words; i += blockDim.x) smem[i] = 0ull;
+            __syncthreads();
+            for (int64_t b = pb; b < pe; b += blockDim.x) {
+                const int64_t r = b + threadIdx.x;
+                const bool valid = r < pe;
+                const uint64_t kraw = valid ? dev::load_raw(skeys, KB, r) : 0ull;
+                uint64_t cv[NV];
+#pragma unroll
+                for (int c = 0; c < NV; ++c)
+                    cv[c] = (valid && spec.nvcols > 0) ? dev::canon(dev::load_raw(svals, vbytes, r), spec.cols[0].conv) : 0ull;
+                const bool sp = lds_row<dev::RtOps, KB, NV>(tab, true, 0, lane, key, spec, kraw, cv, valid, probe_limit, peel_min, hot);
+                if (__ballot(sp) != 0ull) hbm_row<dev::RtOps, KB, NV>(gt, key, spec, sp, kraw, cv, 65, ins, hot);
+            }
+            __syncthreads();
+            using KT = typename LdsKey<KB>::T;
+            auto slot_key = [&](int sl, KT& kw) {
+                kw = 0;
+                if (sl < tab.cr) {
+                    kw = tab.keys[sl];
+                    return kw != 0;
+                }
+                return sl == tab.cr && tab.side[0] != 0ull;
+            };
+            const int wv = threadIdx.x >> 6;
+            const int nw = (blockDim.x + 63) >> 6;
+            uint32_t wave_groups = 0;
+            for (int base = 0; base <= tab.cr; base += blockDim.x) {
+                KT kw;
+                wave_groups += (uint32_t)__popcll(__ballot(slot_key(base + threadIdx.x, kw)));
+            }
+            if (lane == 0) sh[wv] = wave_groups;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                unsigned long long tot = 0;
+                for (int i = 0; i < nw; ++i) {
+                    const unsigned long long c = sh[i];
+                    sh[i] = tot;
+                    tot += c;
+                }
+                sh[16] = tot ? atomicAdd(&counts[w.q], tot) : 0ull;
+            }
+            __syncthreads();
+            int64_t pos = (int64_t)(sh[16] + sh[wv]);
+            for (int base = 0; base <= tab.cr; base += blockDim.x) {
+                const int sl = base + threadIdx.x;
+                KT kw;
+                const bool occ = slot_key(sl, kw);
+                const uint64_t m = __ballot(occ);
+                if (occ) {
+                    const int64_t idx = pos + __popcll(m & dev::lanemask_lt());
+                    if (idx < q.capacity) emit_batch(bo, q, idx, lds_to_enc64<KB>(kw, key_signed), tab.st + sl, (int64_t)tab.T);
+                    else atomicOr(&qstatus[w.q], kStCapacity);
+                }
+                pos += __popcll(m);
+            }
+            __syncthreads();  // LDS, sh and s_item are reused by the next item
+            continue;
+        }
         key.data = q.keys;
         const bool use_lds = q.use_lds != 0;
         if (use_lds) {
@@ -177,41 +405,6 @@ __global__ __launch_bounds__(kBatchBlock) void batch_agg_kernel(const BQuery* __
     }
 }
 
-__device__ __forceinline__ void emit_batch(const BOut& bo, const BQuery& q, int64_t g, uint64_t key_enc,
-                                           const unsigned long long* st)
-{
-    dev::store_bytes(q.out_keys, bo.key_bytes, g, dev::dec64(key_enc));
-#pragma unroll
-    for (int j = 0; j < EG_BATCH_MAX_AGGS; ++j) {
-        if (j >= bo.nouts) break;
-        void* d = q.out_aggs[j];
-        const uint64_t s = st[bo.op[j]];
-        switch (bo.code[j]) {
-        case OUT_U64: dev::store_bytes(d, 8, g, s); break;
-        case OUT_F32: ((float*)d)[g] = (float)__longlong_as_double((long long)s); break;
-        case OUT_F64: ((double*)d)[g] = __longlong_as_double((long long)s); break;
-        case OUT_MAX_I: dev::store_bytes(d, bo.bytes[j], g, s ^ dev::kSign64); break;
-        case OUT_MIN_I: dev::store_bytes(d, bo.bytes[j], g, (~s) ^ dev::kSign64); break;
-        case OUT_MAX_U: dev::store_bytes(d, bo.bytes[j], g, s); break;
-        case OUT_MIN_U: dev::store_bytes(d, bo.bytes[j], g, ~s); break;
-        case OUT_MAX_F: dev::store_double_as(d, bo.bytes[j], g, dev::unord_f64(s)); break;
-        case OUT_MIN_F: dev::store_double_as(d, bo.bytes[j], g, dev::unord_f64(~s)); break;
-        case OUT_ANY:
-            if (bo.is_float[j] && bo.bytes[j] == 4) ((float*)d)[g] = (float)__longlong_as_double((long long)s);
-            else dev::store_bytes(d, bo.bytes[j], g, s);
-            break;
-        default: {
-            const double cnt = (double)(uint64_t)st[bo.op2[j]];
-            const double sum = (bo.code[j] == OUT_AVG_I) ? (double)(int64_t)s
-                             : (bo.code[j] == OUT_AVG_U) ? (double)s
-                             : __longlong_as_double((long long)s);
-            ((double*)d)[g] = sum / cnt;
-            break;
-        }
-        }
-    }
-}
-
 __global__ void batch_compact_kernel(const BQuery* __restrict__ qs, const BCItem* __restrict__ citems,
                                      unsigned long long* metas, unsigned long long* counts, uint32_t* qstatus,
                                      BOut bo, int W)
@@ -238,7 +431,7 @@ __global__ void batch_compact_kernel(const BQuery* __restrict__ qs, const BCItem
         }
         const int64_t idx = dev::block_claim(&counts[c.q], occ, sh);
         if (occ) {
-            if (idx < q.capacity) emit_batch(bo, q, idx, enc, slot + 1);
+            if (idx < q.capacity) emit_batch(bo, q, idx, enc, slot + 1, 1);
             else atomicOr(&qstatus[c.q], kStCapacity);
         }
     }
@@ -263,7 +456,8 @@ struct BatchPlan {
     int W = 0;
     eg_batch_tuning tuning{};
     DevBuf qs, items, citems, tables, metas;  // metas: per-query meta + counts + status + queue
-    int n_items = 0, n_citems = 0;
+    DevBuf ptiles, pqs, hist, skeys, svals;     // the partition pass (radix_bits > 0)
+    int n_items = 0, n_citems = 0, n_ptiles = 0, n_pq = 0;
     uint64_t table_words = 0;
     unsigned long long* counts = nullptr;
     uint32_t* qstatus = nullptr;
@@ -290,6 +484,7 @@ void batch_default_tuning(eg_batch_tuning* t)
     t->lds_slots_log2 = 11;
     t->lds_max_bound_log2 = 11;
     t->blocks_per_cu = 1;
+    t->radix_bits = 12;
 }
 
 BatchPlan* batch_plan_create(const eg_batch_query* queries, int64_t nq, int32_t key_type, int32_t val_type,
@@ -305,7 +500,7 @@ BatchPlan* batch_plan_create(const eg_batch_query* queries, int64_t nq, int32_t 
     else batch_default_tuning(&p->tuning);
     const eg_batch_tuning& T = p->tuning;
     if (T.tile_rows_log2 < 10 || T.tile_rows_log2 > 26 || T.lds_slots_log2 < 6 || T.lds_slots_log2 > 13 ||
-        T.blocks_per_cu < 1 || T.blocks_per_cu > 8)
+        T.blocks_per_cu < 1 || T.blocks_per_cu > 8 || T.radix_bits < 0 || T.radix_bits > kMaxPartBits)
         raise(EG_ERR_INVALID, "batch tuning out of range");
 
     // Lower the requests once: every query shares the key/value types.
@@ -336,15 +531,28 @@ BatchPlan* batch_plan_create(const eg_batch_query* queries, int64_t nq, int32_t 
         p->bo.is_float[j] = is_float(plan.outs[j].type) ? 1 : 0;
     }
 
+    // LDS table of the aggregation kernel (slots clamped to what fits)
+    const DeviceInfo& di = device_info();
+    int cr = T.lds_slots_log2;
+    auto lds_of = [&](int c) { return (size_t)lds::lds_table_words(p->spec.nops, c, 0, p->kb) * 8; };
+    while (cr > 6 && lds_of(cr) + 256 > (size_t)di.max_lds) --cr;
+    p->tuning.lds_slots_log2 = cr;
+    p->lds = lds_of(cr);
+    const int64_t per_part = (int64_t)1 << (cr - 1);  // groups per partition: half a table
+
     // Per-query tables, work items and compaction ranges.
     const int W = p->W;
     const int64_t tile = (int64_t)1 << T.tile_rows_log2;
     const int64_t lds_bound = (int64_t)1 << T.lds_max_bound_log2;
     std::vector<BQuery> hq((size_t)nq);
     std::vector<BItem> hitems;
+    std::vector<int64_t> item_rows;  // estimated rows per item (longest first)
     std::vector<BCItem> hc;
+    std::vector<BPTile> htiles;
+    std::vector<int32_t> hpq;
     std::vector<uint64_t> offs((size_t)nq);
     uint64_t words = 0;
+    int64_t hist_entries = 0, scratch_rows = 0;
     constexpr uint64_t kCompactRange = 1u << 16;
     for (int64_t i = 0; i < nq; ++i) {
         const eg_batch_query& u = queries[i];
@@ -354,7 +562,16 @@ BatchPlan* batch_plan_create(const eg_batch_query* queries, int64_t nq, int32_t 
         for (int j = 0; j < p->bo.nouts; ++j)
             if (u.capacity > 0 && !u.out_aggs[j]) raise(EG_ERR_INVALID, "null batch query output");
         const int64_t bound = std::max<int64_t>(1, std::min<int64_t>(u.group_bound > 0 ? u.group_bound : u.rows, std::max<int64_t>(u.rows, 1)));
-        const int64_t slots = (int64_t)1 << std::max(4, ceil_pow2_log2(2 * bound));
+        // above the LDS bound: partitioned (its HBM table only takes the keys
+        // a partition's LDS table cannot hold), or per-row HBM-table atomics
+        const bool part = T.radix_bits > 0 && bound > lds_bound && u.rows > 0 && u.rows < ((int64_t)1 << 31);
+        const int pbits = part ? std::min(T.radix_bits, std::max(1, ceil_pow2_log2((bound + per_part - 1) / per_part))) : 0;
+        // a partitioned query's HBM table takes what its partitions' LDS
+        // tables cannot: the groups beyond half a table per partition (when
+        // radix_bits caps the partition count), plus an eighth of the bound
+        const int64_t excess = part ? std::max<int64_t>(0, bound - (per_part << pbits)) : 0;
+        const int64_t slots = part ? (int64_t)1 << std::max(6, ceil_pow2_log2(2 * (excess + bound / 8)))
+                                   : (int64_t)1 << std::max(4, ceil_pow2_log2(2 * bound));
         BQuery& q = hq[(size_t)i];
         std::memset(&q, 0, sizeof q);
         q.keys = u.keys;
@@ -367,13 +584,39 @@ BatchPlan* batch_plan_create(const eg_batch_query* queries, int64_t nq, int32_t 
         q.use_lds = bound <= lds_bound ? 1 : 0;
         offs[(size_t)i] = words;
         words += (uint64_t)(slots + 1) * W;
-        for (int64_t b = 0; b < u.rows; b += tile) hitems.push_back(BItem{(int32_t)i, 0, b, std::min(u.rows, b + tile)});
+        if (part) {
+            q.pbits = pbits;
+            q.ntiles = (u.rows + kPTile - 1) / kPTile;
+            q.hist_off = hist_entries;
+            q.scratch_off = scratch_rows;
+            hist_entries += ((int64_t)1 << q.pbits) * q.ntiles;
+            scratch_rows += u.rows;
+            hpq.push_back((int32_t)i);
+            for (int64_t t = 0; t < q.ntiles; ++t) htiles.push_back(BPTile{(int32_t)i, (int32_t)t});
+            for (int64_t d = 0; d < ((int64_t)1 << q.pbits); ++d) {
+                hitems.push_back(BItem{(int32_t)i, 1, d, d + 1});
+                item_rows.push_back(u.rows >> q.pbits);
+            }
+        } else {
+            for (int64_t b = 0; b < u.rows; b += tile) {
+                hitems.push_back(BItem{(int32_t)i, 0, b, std::min(u.rows, b + tile)});
+                item_rows.push_back(std::min(u.rows, b + tile) - b);
+            }
+        }
         for (uint64_t b = 0; b < (uint64_t)slots + 1; b += kCompactRange)
             hc.push_back(BCItem{(int32_t)i, 0, b, std::min<uint64_t>((uint64_t)slots + 1, b + kCompactRange)});
     }
     // Longest items first keeps the tail of the persistent grid short.
-    std::stable_sort(hitems.begin(), hitems.end(),
-                     [](const BItem& a, const BItem& b) { return (a.end - a.begin) > (b.end - b.begin); });
+    {
+        std::vector<size_t> order(hitems.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return item_rows[a] > item_rows[b]; });
+        std::vector<BItem> sorted(hitems.size());
+        for (size_t k = 0; k < order.size(); ++k) sorted[k] = hitems[order[k]];
+        hitems.swap(sorted);
+    }
+    if (hist_entries >= ((int64_t)1 << 40) || htiles.size() >= ((size_t)1 << 31))
+        raise(EG_ERR_UNSUPPORTED, "batch partition pass too large");
     if (hitems.size() >= ((size_t)1 << 31) || hc.size() >= ((size_t)1 << 31)) raise(EG_ERR_UNSUPPORTED, "batch too large");
     p->n_items = (int)hitems.size();
     p->n_citems = (int)hc.size();
@@ -398,14 +641,19 @@ BatchPlan* batch_plan_create(const eg_batch_query* queries, int64_t nq, int32_t 
     if (!hitems.empty())
         EG_HIP(hipMemcpyAsync(p->items.get(), hitems.data(), hitems.size() * sizeof(BItem), hipMemcpyHostToDevice, s));
     EG_HIP(hipMemcpyAsync(p->citems.get(), hc.data(), hc.size() * sizeof(BCItem), hipMemcpyHostToDevice, s));
+    p->n_ptiles = (int)htiles.size();
+    p->n_pq = (int)hpq.size();
+    if (!htiles.empty()) {
+        p->ptiles = DevBuf(htiles.size() * sizeof(BPTile), s);
+        p->pqs = DevBuf(hpq.size() * sizeof(int32_t), s);
+        p->hist = DevBuf((size_t)hist_entries * 4, s);
+        p->skeys = DevBuf((size_t)scratch_rows * p->kb, s);
+        if (uses_vals) p->svals = DevBuf((size_t)scratch_rows * p->vbytes, s);
+        EG_HIP(hipMemcpyAsync(p->ptiles.get(), htiles.data(), htiles.size() * sizeof(BPTile), hipMemcpyHostToDevice, s));
+        EG_HIP(hipMemcpyAsync(p->pqs.get(), hpq.data(), hpq.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    }
     EG_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
 
-    const DeviceInfo& di = device_info();
-    int cr = T.lds_slots_log2;
-    auto lds_of = [&](int c) { return (size_t)lds::lds_table_words(p->spec.nops, c, 0, p->kb) * 8; };
-    while (cr > 6 && lds_of(cr) + 256 > (size_t)di.max_lds) --cr;
-    p->tuning.lds_slots_log2 = cr;
-    p->lds = lds_of(cr);
     p->grid = di.num_cus * T.blocks_per_cu;
     // raise the kernel's LDS limit now: no attribute calls during graph capture
     const void* kern = p->kb == 4 ? (const void*)batch_agg_kernel<4, 1> : (const void*)batch_agg_kernel<8, 1>;
@@ -419,7 +667,23 @@ void launch_batch_agg(BatchPlan& p, hipStream_t s)
     auto kern = batch_agg_kernel<KB, NV>;
     hipLaunchKernelGGL(kern, dim3(p.grid), dim3(kBatchBlock), p.lds, s, p.qs.get<BQuery>(), p.items.get<BItem>(),
                        p.n_items, p.next_item, p.metas.get<unsigned long long>(), p.spec, is_signed_int(p.key_type) ? 1 : 0,
-                       p.vbytes, p.W, p.tuning.lds_slots_log2, 16, 8);
+                       p.vbytes, p.W, p.tuning.lds_slots_log2, 16, 8, (const uint32_t*)p.hist.get<uint32_t>(),
+                       (const void*)p.skeys.get(), (const void*)p.svals.get(), p.counts, p.qstatus, p.bo);
+    EG_LAUNCH_CHECK();
+}
+
+template <int KB>
+void launch_partition_pass(BatchPlan& p, hipStream_t s)
+{
+    hipLaunchKernelGGL(bpart_hist_kernel<KB>, dim3((unsigned)p.n_ptiles), dim3(kPB), 0, s, p.qs.get<BQuery>(),
+                       p.ptiles.get<BPTile>(), p.hist.get<uint32_t>());
+    EG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bpart_scan_kernel, dim3((unsigned)p.n_pq), dim3(1024), 0, s, p.qs.get<BQuery>(),
+                       p.pqs.get<int32_t>(), p.hist.get<uint32_t>());
+    EG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bpart_scatter_kernel<KB>, dim3((unsigned)p.n_ptiles), dim3(kPB), 0, s, p.qs.get<BQuery>(),
+                       p.ptiles.get<BPTile>(), (const uint32_t*)p.hist.get<uint32_t>(), p.skeys.get(), p.svals.get(),
+                       p.vbytes);
     EG_LAUNCH_CHECK();
 }
 
@@ -435,6 +699,11 @@ void batch_run(BatchPlan& p, hipStream_t s)
     };
     fill(p.tables.get<unsigned long long>(), p.table_words);
     fill(p.metas.get<unsigned long long>(), p.meta_words);
+    if (p.n_ptiles > 0) {
+        KernelTimer timer("batch_partition", s);
+        if (p.kb == 4) launch_partition_pass<4>(p, s);
+        else launch_partition_pass<8>(p, s);
+    }
     {
         KernelTimer timer("batch_agg", s);
         if (p.kb == 4) launch_batch_agg<4, 1>(p, s);

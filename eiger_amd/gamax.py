@@ -616,11 +616,13 @@ class batch_query:
         self.capacity = int(out_keys.size if capacity is None else capacity)
 
 
-def batch_tuning(tile_rows_log2=None, lds_slots_log2=None, lds_max_bound_log2=None, blocks_per_cu=None):
+def batch_tuning(tile_rows_log2=None, lds_slots_log2=None, lds_max_bound_log2=None, blocks_per_cu=None,
+                 radix_bits=None):
     t = _lib_mod.BatchTuning()
     lib().eg_batch_tuning_default(C.byref(t))
     for k, v in (("tile_rows_log2", tile_rows_log2), ("lds_slots_log2", lds_slots_log2),
-                 ("lds_max_bound_log2", lds_max_bound_log2), ("blocks_per_cu", blocks_per_cu)):
+                 ("lds_max_bound_log2", lds_max_bound_log2), ("blocks_per_cu", blocks_per_cu),
+                 ("radix_bits", radix_bits)):
         if v is not None:
             setattr(t, k, int(v))
     return t

@@ -333,6 +333,9 @@ typedef struct eg_batch_tuning {
     int32_t lds_slots_log2;      /* LDS table slots per workgroup (6..13) */
     int32_t lds_max_bound_log2;  /* queries with group_bound <= 2^this use the LDS table */
     int32_t blocks_per_cu;       /* persistent workgroups per CU (1..8) */
+    int32_t radix_bits;          /* queries above the LDS bound: one radix pass on up to this many
+                                    hash bits (0..12) into partitions of about half an LDS table,
+                                    each aggregated in LDS; 0: per-row atomics on an HBM table */
 } eg_batch_tuning;
 typedef struct eg_batch_plan_s* eg_batch_plan;
 void eg_batch_tuning_default(eg_batch_tuning* t);
@@ -451,7 +454,7 @@ int eg_jit_selftest(const char* arch, char* log, int64_t log_len);
 
 /* Reference copy kernel for bandwidth calibration: copies `bytes` between two
  * scratch buffers `iters` times with 16-byte vector loads/stores (kernel name
- * "probe_copy"); *ms = mean time per copy.  Mirrors the reference's
+ * "probe_copy", four workgroups per CU); *ms = the fastest copy.  Mirrors the reference's
  * read/write bandwidth probe (benchmarks/misc/read_write_nvbench.cu:29-80). */
 int eg_probe_copy(int64_t bytes, int32_t iters, void* stream, double* ms);
 

@@ -49,10 +49,14 @@ SPECS = [(1000, 1, 0.0), (1000, 500, 0.0), (50_000, 17, 0.0), (50_000, 3000, 0.0
          (400_000, 150_000, 0.5), (65_536, 4096, 1.5)]
 
 
+@pytest.mark.parametrize("radix_bits", [0, 4, 12])
 @pytest.mark.parametrize("key_dtype", [np.int32, np.int64])
-def test_batch_matches_oracle_per_query(key_dtype):
+def test_batch_matches_oracle_per_query(key_dtype, radix_bits):
+    """radix_bits 0: queries above the LDS bound use per-row HBM-table
+    atomics; 4 and 12: they are partitioned first (4 bits: partitions far
+    above half an LDS table, so keys spill to the query's HBM table)."""
     host, queries, keep, reqs = _make_batch(SPECS, key_dtype)
-    plan = G.batch_plan(queries, G.to_type_id(key_dtype), T.INT64, reqs)
+    plan = G.batch_plan(queries, G.to_type_id(key_dtype), T.INT64, reqs, tuning=G.batch_tuning(radix_bits=radix_bits))
     plan.run()
     ng, st = plan.results()
     assert np.all(st == 0)
