@@ -270,8 +270,9 @@ __device__ __forceinline__ void emit_batch(const BOut& bo, const BQuery& q, int6
     }
 }
 
+// Eight waves per SIMD, so two workgroups fit a CU (blocks_per_cu = 2).
 template <int KB, int NV>
-__global__ __launch_bounds__(kBatchBlock) void batch_agg_kernel(const BQuery* __restrict__ qs,
+__global__ __launch_bounds__(kBatchBlock, 8) void batch_agg_kernel(const BQuery* __restrict__ qs,
                                                                 const BItem* __restrict__ items, int n_items,
                                                                 unsigned int* next_item, unsigned long long* metas,
                                                                 AggSpec spec, int key_signed, int vbytes, int W,

@@ -73,7 +73,9 @@ __device__ __forceinline__ uint64_t read_leader(uint64_t tag, int leader) { retu
 // must call it, with the full wave active and the same min_lanes.  Otherwise
 // the leader is the lowest pending lane; its tag is read with a scalar lane
 // read.
-template <class OPS, class Tag, class OperandFn, class BaseFn>
+// TRIES: leaders tried after the hot tag (each costs a lane read, a ballot
+// and a population count of scalar work per call).
+template <class OPS, int TRIES = kPeelTries, class Tag, class OperandFn, class BaseFn>
 __device__ __forceinline__ bool wave_peel(bool pending, Tag tag, Tag& hot, const AggSpec& spec, int min_lanes,
                                           OperandFn operand, BaseFn base, int64_t stride)
 {
@@ -102,7 +104,7 @@ __device__ __forceinline__ bool wave_peel(bool pending, Tag tag, Tag& hot, const
         }
     }
 #pragma unroll
-    for (int it = 0; it < kPeelTries; ++it) {
+    for (int it = 0; it < TRIES; ++it) {
         if (pend == 0) break;
         const int leader = __ffsll((unsigned long long)pend) - 1;
         const Tag ltag = read_leader(tag, leader);

@@ -304,6 +304,36 @@ __device__ __forceinline__ void store_double_as(void* p, int bytes, int64_t i, d
     else ((double*)p)[i] = d;
 }
 
+// One output value of group g from its state s (and the count state cnt for
+// AVG); with constant code/bytes/is_float the switch folds away.
+__device__ __forceinline__ void emit_one(int32_t code, int32_t bytes, int32_t is_float, void* data, int64_t g,
+                                         uint64_t s, uint64_t cnt)
+{
+    switch (code) {
+    case OUT_U64: store_bytes(data, 8, g, s); break;
+    case OUT_F32: ((float*)data)[g] = (float)__longlong_as_double((long long)s); break;
+    case OUT_F64: ((double*)data)[g] = __longlong_as_double((long long)s); break;
+    case OUT_MAX_I: store_bytes(data, bytes, g, s ^ kSign64); break;
+    case OUT_MIN_I: store_bytes(data, bytes, g, (~s) ^ kSign64); break;
+    case OUT_MAX_U: store_bytes(data, bytes, g, s); break;
+    case OUT_MIN_U: store_bytes(data, bytes, g, ~s); break;
+    case OUT_MAX_F: store_double_as(data, bytes, g, unord_f64(s)); break;
+    case OUT_MIN_F: store_double_as(data, bytes, g, unord_f64(~s)); break;
+    case OUT_ANY:
+        // canonical value: truncation restores integers, floats come back from double
+        if (is_float && bytes == 4) ((float*)data)[g] = (float)__longlong_as_double((long long)s);
+        else store_bytes(data, bytes, g, s);
+        break;
+    default: {
+        const double sum = (code == OUT_AVG_I) ? (double)(int64_t)s
+                         : (code == OUT_AVG_U) ? (double)s
+                         : __longlong_as_double((long long)s);
+        ((double*)data)[g] = sum / (double)cnt;
+        break;
+    }
+    }
+}
+
 // Writes group `g`: its key (from the 64-bit encoding) and every output
 // column.  States of op a are read at st[a * stride].
 template <typename W>
@@ -313,31 +343,47 @@ __device__ __forceinline__ void emit_group(const OutSpec& os, int64_t g, uint64_
     store_bytes(os.key_data, os.key_bytes, g, dec64(key_enc));
     for (int j = 0; j < os.nouts; ++j) {
         const OutDesc& o = os.outs[j];
-        uint64_t s = (uint64_t)st[(int64_t)o.op * stride];
-        switch (o.code) {
-        case OUT_U64: store_bytes(o.data, 8, g, s); break;
-        case OUT_F32: ((float*)o.data)[g] = (float)__longlong_as_double((long long)s); break;
-        case OUT_F64: ((double*)o.data)[g] = __longlong_as_double((long long)s); break;
-        case OUT_MAX_I: store_bytes(o.data, o.bytes, g, s ^ kSign64); break;
-        case OUT_MIN_I: store_bytes(o.data, o.bytes, g, (~s) ^ kSign64); break;
-        case OUT_MAX_U: store_bytes(o.data, o.bytes, g, s); break;
-        case OUT_MIN_U: store_bytes(o.data, o.bytes, g, ~s); break;
-        case OUT_MAX_F: store_double_as(o.data, o.bytes, g, unord_f64(s)); break;
-        case OUT_MIN_F: store_double_as(o.data, o.bytes, g, unord_f64(~s)); break;
-        case OUT_ANY:
-            // canonical value: truncation restores integers, floats come back from double
-            if (o.is_float && o.bytes == 4) ((float*)o.data)[g] = (float)__longlong_as_double((long long)s);
-            else store_bytes(o.data, o.bytes, g, s);
-            break;
-        default: {
-            double cnt = (double)(uint64_t)st[(int64_t)o.op2 * stride];
-            double sum = (o.code == OUT_AVG_I) ? (double)(int64_t)s
-                       : (o.code == OUT_AVG_U) ? (double)s
-                       : __longlong_as_double((long long)s);
-            ((double*)o.data)[g] = sum / cnt;
-            break;
-        }
-        }
+        const uint64_t s = (uint64_t)st[(int64_t)o.op * stride];
+        const uint64_t cnt = o.op2 >= 0 ? (uint64_t)st[(int64_t)o.op2 * stride] : 0ull;
+        emit_one(o.code, o.bytes, o.is_float, o.data, g, s, cnt);
+    }
+}
+
+// Output-list policies of the partition unit kernel: RtOuts reads the output
+// descriptors at run time (emit_group); FixOuts<W...> has them compiled in,
+// one word per output column: code | bytes << 4 | op << 8 | (op2 + 1) << 16 |
+// is_float << 24 (the run-time compiler instantiates it per request list).
+struct RtOuts {
+    static constexpr bool kFixed = false;
+};
+
+template <uint32_t... W>
+struct FixOuts {
+    static constexpr bool kFixed = true;
+    static constexpr int kN = sizeof...(W);
+    __device__ static constexpr uint32_t word(int j)
+    {
+        constexpr uint32_t w[kN > 0 ? kN : 1] = {W...};
+        return w[j];
+    }
+};
+
+template <class OUTS, typename W>
+__device__ __forceinline__ void emit_outs(const OutSpec& os, int key_bytes, int64_t g, uint64_t key_enc, const W* st,
+                                          int64_t stride)
+{
+    if constexpr (!OUTS::kFixed) {
+        emit_group(os, g, key_enc, st, stride);
+    } else {
+        store_bytes(os.key_data, key_bytes, g, dec64(key_enc));
+        static_for_seq(std::make_integer_sequence<int, OUTS::kN>{}, [&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            constexpr uint32_t w = OUTS::word(j);
+            constexpr int32_t op = (int32_t)((w >> 8) & 0xff), op2 = (int32_t)((w >> 16) & 0xff) - 1;
+            const uint64_t s = (uint64_t)st[(int64_t)op * stride];
+            const uint64_t cnt = op2 >= 0 ? (uint64_t)st[(int64_t)op2 * stride] : 0ull;
+            emit_one((int32_t)(w & 15), (int32_t)((w >> 4) & 15), (int32_t)((w >> 24) & 1), os.outs[j].data, g, s, cnt);
+        });
     }
 }
 

@@ -42,13 +42,10 @@ namespace {
 
 using namespace lds;
 
-// Digit width of one partitioning pass.  Fan-out 64 keeps the rows of one
-// digit in runs long enough for full-line stores (measured: a 6-bit pass on
-// 2^28 rows takes about half the time of an 8-bit one).
-constexpr int kMaxDigitBits = 6;
+// Widest digit of a partitioning pass (fan-out 256); EG_PARTITION_DIGIT_BITS
+// (4..8) lowers it for tuning.  Passes over the same bits are balanced.
+constexpr int kMaxDigitBits = 8;
 
-// Widest digit of a partitioning pass; EG_PARTITION_DIGIT_BITS (4..8)
-// overrides it for tuning.
 int max_digit_bits()
 {
     static const int v = [] {
@@ -59,22 +56,24 @@ int max_digit_bits()
     return v;
 }
 
-// Workgroup size and LDS table slots (log2) of the unit kernel; overridable
-// for tuning with EG_PARTITION_TUNE="slots_log2,threads".
+// Unit kernel geometry: 1024-thread workgroups whose LDS table takes what
+// leaves room for two workgroups per CU (160 KB of LDS: about 79 KB each,
+// e.g. 1788 slots for the five 8-byte states and 4-byte keys of the
+// benchmark), filled to 2/3 by design.  EG_PARTITION_TUNE="slots,threads"
+// fixes both for tuning (slots a multiple of 4).
 struct PartTune {
-    int cr_log2 = 11;
+    int cr = 0;  // 0: from the LDS budget
     int block = 1024;
-    bool fixed = false;  // set by the override
 };
 PartTune part_tune()
 {
     PartTune t;
     if (const char* e = std::getenv("EG_PARTITION_TUNE")) {
         int a = 0, b = 0;
-        if (std::sscanf(e, "%d,%d", &a, &b) == 2 && a >= 6 && a <= 12 && (b == 256 || b == 512 || b == 1024)) {
-            t.cr_log2 = a;
+        if (std::sscanf(e, "%d,%d", &a, &b) == 2 && a >= 64 && a <= 16384 && a % 4 == 0 &&
+            (b == 256 || b == 512 || b == 1024)) {
+            t.cr = a;
             t.block = b;
-            t.fixed = true;
         }
     }
     return t;
@@ -83,14 +82,19 @@ PartTune part_tune()
 using agg::kPBMax;
 using agg::Unit;
 
+using agg::OutClaims;
+
+// Eight waves per SIMD: two 1024-thread workgroups per CU, which the LDS
+// table is sized for (without the bound the compiler's scalar registers
+// alone leave room for only one).
 template <int KB, class VL, bool GATHER>
-__global__ __launch_bounds__(kPBMax) void part_agg_kernel(KeyDesc key, AggSpec spec, const int64_t* perm,
+__global__ __launch_bounds__(kPBMax, 8) void part_agg_kernel(KeyDesc key, AggSpec spec, const int64_t* perm,
                                                        const Unit* units, GTable gt, OutSpec os,
-                                                       unsigned long long* counter, int64_t capacity,
-                                                       int cr_log2, int probe_limit, int peel_min)
+                                                       OutClaims claims, int64_t capacity,
+                                                       int cr, int probe, int peel_min)
 {
-    agg::part_agg<KB, VL, dev::RtOps, GATHER>(key, spec, perm, units, gt, os, counter, capacity, cr_log2,
-                                              probe_limit, peel_min);
+    agg::part_agg<KB, VL, dev::RtOps, GATHER>(key, spec, perm, units, gt, os, claims, capacity, cr, probe,
+                                              peel_min);
 }
 
 template <int KB>
@@ -141,6 +145,27 @@ __global__ void part_compact_kernel(GTable gt, OutSpec os, unsigned long long* c
     }
 }
 
+// Output rows past the group count moved into the holes below it (the
+// unused tails of the lanes' last chunks): op i copies len[i] rows from
+// src[i] to dst[i] in every output column; sources and destinations are
+// disjoint (every source is >= the group count, every destination below it).
+constexpr int kMaxMoves = 32;
+struct MoveList {
+    int64_t src[kMaxMoves], dst[kMaxMoves], len[kMaxMoves];
+};
+
+__global__ void move_rows_kernel(OutSpec os, MoveList ml)
+{
+    const int op = blockIdx.y;
+    const int64_t len = ml.len[op];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = ml.src[op] + i, d = ml.dst[op] + i;
+        dev::store_bytes(os.key_data, os.key_bytes, d, dev::load_raw(os.key_data, os.key_bytes, s));
+        for (int j = 0; j < os.nouts; ++j)
+            dev::store_bytes(os.outs[j].data, os.outs[j].bytes, d, dev::load_raw(os.outs[j].data, os.outs[j].bytes, s));
+    }
+}
+
 int ceil_log2(int64_t x)
 {
     int b = 0;
@@ -150,7 +175,7 @@ int ceil_log2(int64_t x)
 
 template <int KB, bool GATHER>
 void launch_part_agg(const KeyDesc& key, const AggSpec& spec, const int64_t* perm, const Unit* units, int nunits,
-                     const GTable& gt, const OutSpec& os, unsigned long long* counter, int64_t capacity, int cr_log2,
+                     const GTable& gt, const OutSpec& os, const OutClaims& claims, int64_t capacity, int cr,
                      size_t lds, int block, int64_t rows, hipStream_t s)
 {
     with_vlayout(spec, true, [&](auto vl) {
@@ -158,39 +183,39 @@ void launch_part_agg(const KeyDesc& key, const AggSpec& spec, const int64_t* per
         auto kern = part_agg_kernel<KB, VL, GATHER>;
         if (enable_dynamic_lds((const void*)kern) < lds) raise(EG_ERR_UNSUPPORTED, "partition LDS table too large");
         KernelTimer timer("partition_agg", s);
-        int probe_limit = 64, peel_min = default_peel_min();
+        int probe = 16, peel_min = default_peel_min();  // probe: buckets of four slots
         // the kernel with this request's ops compiled in, once it is available
         const jit::Kind kind = GATHER ? jit::Kind::PART_AGG_GATHER : jit::Kind::PART_AGG;
-        if (hipFunction_t f = jit::lookup(kind, KB, jit::vlayout_name<VL>(), spec, rows)) {
+        if (hipFunction_t f = jit::lookup(kind, KB, jit::vlayout_name<VL>(), spec, rows, &os)) {
             KeyDesc a_key = key;
             AggSpec a_spec = spec;
             const int64_t* a_perm = perm;
             const Unit* a_units = units;
             GTable a_gt = gt;
             OutSpec a_os = os;
-            unsigned long long* a_counter = counter;
+            OutClaims a_claims = claims;
             int64_t a_cap = capacity;
-            int a_cr = cr_log2;
-            void* args[] = {&a_key, &a_spec, &a_perm, &a_units, &a_gt, &a_os, &a_counter, &a_cap, &a_cr,
-                            &probe_limit, &peel_min};
+            int a_cr = cr;
+            void* args[] = {&a_key, &a_spec, &a_perm, &a_units, &a_gt, &a_os, &a_claims, &a_cap, &a_cr,
+                            &probe, &peel_min};
             if (jit::launch(f, (unsigned)nunits, (unsigned)block, lds, s, args)) return;
         }
-        hipLaunchKernelGGL(kern, dim3(nunits), dim3(block), lds, s, key, spec, perm, units, gt, os, counter, capacity,
-                           cr_log2, probe_limit, peel_min);
+        hipLaunchKernelGGL(kern, dim3(nunits), dim3(block), lds, s, key, spec, perm, units, gt, os, claims, capacity,
+                           cr, probe, peel_min);
         EG_LAUNCH_CHECK();
     });
 }
 
 template <int KB>
 void launch_part(bool gather, const KeyDesc& key, const AggSpec& spec, const int64_t* perm, const Unit* units,
-                 int nunits, const GTable& gt, const OutSpec& os, unsigned long long* counter, int64_t capacity,
-                 int cr_log2, size_t lds, int block, int64_t rows, hipStream_t s)
+                 int nunits, const GTable& gt, const OutSpec& os, const OutClaims& claims, int64_t capacity,
+                 int cr, size_t lds, int block, int64_t rows, hipStream_t s)
 {
     if (gather)
-        launch_part_agg<KB, true>(key, spec, perm, units, nunits, gt, os, counter, capacity, cr_log2, lds, block,
+        launch_part_agg<KB, true>(key, spec, perm, units, nunits, gt, os, claims, capacity, cr, lds, block,
                                   rows, s);
     else
-        launch_part_agg<KB, false>(key, spec, perm, units, nunits, gt, os, counter, capacity, cr_log2, lds, block,
+        launch_part_agg<KB, false>(key, spec, perm, units, nunits, gt, os, claims, capacity, cr, lds, block,
                                    rows, s);
 }
 
@@ -204,38 +229,22 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     const int nops = plan.spec.nops;
     const int64_t bound = std::max<int64_t>(1, std::min<int64_t>(group_bound, n));
 
-    // LDS table and workgroup: 1024 slots with 512 threads runs the unit
-    // kernel about 1.5x faster than 2048 slots with 1024 threads (three
-    // resident workgroups per CU instead of one), but doubles the partition
-    // count; take it when that costs no extra radix pass, widening the digits
-    // from 6 to 7 bits if that is what it takes (a 7-bit pass costs ~7% more
-    // than a 6-bit one; 2^28 rows, D = 2^22: 6.4 ms against 6.7 ms).
-    PartTune tune = part_tune();
-    auto bits_for = [&](int c) {
-        const int64_t pp = (int64_t)1 << (c - 1);
-        return std::min(24, std::max(1, ceil_log2((bound + pp - 1) / pp)));
-    };
-    int digit_bits = max_digit_bits();
-    if (!tune.fixed) {
-        auto passes = [&](int c, int mb) { return (bits_for(c) + mb - 1) / mb; };
-        const int p11 = passes(11, digit_bits);
-        if (bound > ((int64_t)1 << 12)) {
-            if (passes(10, digit_bits) == p11) {
-                tune.cr_log2 = 10;
-                tune.block = 512;
-            } else if (digit_bits < 7 && passes(10, 7) == p11) {
-                tune.cr_log2 = 10;
-                tune.block = 512;
-                digit_bits = 7;
-            }
-        }
+    // LDS table and workgroup (see PartTune); the partition count is chosen
+    // so a partition holds about 2/3 of the table.
+    const PartTune tune = part_tune();
+    const size_t static_lds = 256;
+    auto lds_of = [&](int c) { return (size_t)agg::unit_table_words(nops, c, kb) * 8; };
+    int cr = tune.cr;
+    if (cr == 0) {
+        const size_t budget = std::min<size_t>((size_t)di.max_lds, 160 * 1024) / 2 - static_lds;
+        cr = 4;
+        while (cr < 16384 && lds_of(cr + 4) <= budget) cr += 4;
     }
-    int cr_log2 = tune.cr_log2;
-    auto lds_of = [&](int c) { return (size_t)lds_table_words(nops, c, 0, kb) * 8; };
-    while (cr_log2 > 6 && lds_of(cr_log2) + 256 > (size_t)di.max_lds) --cr_log2;  // 256: static LDS
-    const size_t lds = lds_of(cr_log2);
-    const int64_t per_part = (int64_t)1 << (cr_log2 - 1);  // target groups per partition (load 1/2)
+    while (cr > 64 && lds_of(cr) + static_lds > (size_t)di.max_lds) cr -= 4;
+    const size_t lds = lds_of(cr);
+    const int64_t per_part = std::max<int64_t>(1, (int64_t)cr * 2 / 3);  // target groups per partition
     const int bits = std::min(24, std::max(1, ceil_log2((bound + per_part - 1) / per_part)));
+    const int digit_bits = max_digit_bits();
     const uint32_t pmask = (uint32_t)((1u << bits) - 1u);
 
     std::unique_ptr<HostPhase> phase_prep(new HostPhase("host_partition_prep", s));
@@ -360,46 +369,112 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     const int64_t spill_groups = std::min<int64_t>(bound, sliced_parts * per_part * 2) + bound / 32;
     while (slots < 2 * spill_groups) slots <<= 1;
     const int W = table_words_per_slot(nops);
+    // Output chunk of the lane claims (see agg::OutClaims): a power of two
+    // above a unit's largest group count (its slots + the side slot), and at
+    // least 1/256 of the bound (up to 2^16 rows) so few chunks are claimed.
+    int64_t chunk = 4096;
+    while (chunk < (int64_t)cr + 1) chunk <<= 1;
+    while (chunk < 65536 && chunk * 256 < bound) chunk <<= 1;
     for (;;) {
+        // room for the unfilled tails of the lanes' last chunks
+        const int64_t cap_rows = capacity + (int64_t)agg::kClaimLanes * chunk;
         Result res;
         OutSpec os;
-        alloc_outputs(plan, capacity, s, res, os);
+        alloc_outputs(plan, cap_rows, s, res, os);
         HbmTable ht(slots, W, s);
-        DevBuf counter(8, s);
-        fill_zero(counter.get(), 8, s);
+        OutClaims claims;
+        claims.chunk = chunk;
+        claims.maxc = cap_rows / chunk + 2;
+        const size_t claim_bytes = (size_t)agg::claim_words(claims.maxc) * 8;
+        DevBuf cbuf(claim_bytes, s);
+        fill_zero(cbuf.get(), claim_bytes, s);
+        claims.words = cbuf.get<unsigned long long>();
         if (kb == 4)
-            launch_part<4>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
-                           counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, n, s);
+            launch_part<4>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os, claims,
+                           cap_rows, cr, lds, tune.block, n, s);
         else
-            launch_part<8>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os,
-                           counter.get<unsigned long long>(), capacity, cr_log2, lds, tune.block, n, s);
+            launch_part<8>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os, claims,
+                           cap_rows, cr, lds, tune.block, n, s);
         int64_t meta[3];
         ht.read_meta(meta, s);
         const bool table_full = meta[2] != 0;
         if (!table_full) {
-            // append HBM-table groups
+            // append HBM-table groups after every claimed chunk
             const uint64_t total = ht.gt.mask + 2;
             const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + 255) / 256), (int64_t)di.num_cus * 8));
             KernelTimer timer("partition_compact", s);
             hipLaunchKernelGGL(part_compact_kernel, dim3((unsigned)grid), dim3(256), 0, s, ht.gt, os,
-                               counter.get<unsigned long long>(), capacity);
+                               claims.words, cap_rows);
             EG_LAUNCH_CHECK();
         }
-        int64_t* h = pinned_i64(1);
-        EG_HIP(hipMemcpyAsync(h, counter.get(), 8, hipMemcpyDeviceToHost, s));
+        int64_t* h = (int64_t*)pinned_staging((size_t)agg::kClaimHeadWords * 8, 2);
+        EG_HIP(hipMemcpyAsync(h, claims.words, (size_t)agg::kClaimHeadWords * 8, hipMemcpyDeviceToHost, s));
         EG_HIP(hipStreamSynchronize(s));
-        const int64_t groups = h[0];
-        if (table_full || groups > capacity) {
+        const int64_t total = h[0];  // output rows claimed: chunks, then the HBM table's groups
+        if (table_full || total > cap_rows) {
             // the HBM table or the output ran out: retry with room for every row
             if (capacity >= n && slots >= 2 * n) raise(EG_ERR_CAPACITY, "partition group-by overflow");
             timing_count(table_full ? "partition_retry_table" : "partition_retry_capacity");
-            if (groups > capacity) capacity = n;
+            if (total > cap_rows) capacity = n;
             if (table_full) {
                 int64_t ns = slots;
                 while (ns < 2 * std::max<int64_t>(bound, meta[0] * 2)) ns <<= 1;
                 slots = std::max<int64_t>(ns, slots * 4);
             }
             continue;
+        }
+        // Rows written: every lane's logical rows (its chunks, the last one
+        // partly used) plus the HBM table's groups after the chunks.  Rows at
+        // or past the group count move into the holes below it.
+        int64_t fills = 0, chunk_rows = 0;
+        std::vector<std::pair<int64_t, int64_t>> holes;
+        for (int x = 0; x < agg::kClaimLanes; ++x) {
+            const int64_t f = h[32 * (1 + x)];
+            const int64_t nch = (f + chunk - 1) / chunk;
+            fills += f;
+            chunk_rows += nch * chunk;
+            if (f % chunk != 0) {
+                const uint64_t last = (uint64_t)h[1 + x];
+                if ((int64_t)(last >> 40) != nch - 1) raise(EG_ERR_DEVICE, "partition output claims inconsistent");
+                const int64_t base = (int64_t)(last & ((1ull << 40) - 1));
+                holes.emplace_back(base + f % chunk, base + chunk);
+            }
+        }
+        const int64_t groups = fills + (total - chunk_rows);
+        if (total < chunk_rows || groups > capacity + (int64_t)agg::kClaimLanes * chunk)
+            raise(EG_ERR_DEVICE, "partition output claims inconsistent");
+        std::sort(holes.begin(), holes.end());
+        std::vector<std::pair<int64_t, int64_t>> dst, src;  // [begin, end) runs
+        int64_t at = groups;
+        for (const auto& hl : holes) {
+            if (hl.first < groups) dst.emplace_back(hl.first, std::min(hl.second, groups));
+            if (hl.second <= at) continue;
+            if (hl.first > at) src.emplace_back(at, std::min(hl.first, total));
+            at = std::max(at, hl.second);
+        }
+        if (at < total) src.emplace_back(at, total);
+        MoveList ml{};
+        int nm = 0;
+        int64_t longest = 0;
+        size_t si = 0, di2 = 0;
+        int64_t so = 0, doff = 0;
+        while (di2 < dst.size() && si < src.size()) {
+            const int64_t len = std::min(dst[di2].second - dst[di2].first - doff, src[si].second - src[si].first - so);
+            if (nm == kMaxMoves) raise(EG_ERR_DEVICE, "partition output holes: too many moves");
+            ml.src[nm] = src[si].first + so;
+            ml.dst[nm] = dst[di2].first + doff;
+            ml.len[nm] = len;
+            longest = std::max(longest, len);
+            ++nm;
+            doff += len;
+            so += len;
+            if (doff == dst[di2].second - dst[di2].first) { ++di2; doff = 0; }
+            if (so == src[si].second - src[si].first) { ++si; so = 0; }
+        }
+        if (nm > 0) {
+            const unsigned gx = (unsigned)std::min<int64_t>((longest + 255) / 256, 1024);
+            hipLaunchKernelGGL(move_rows_kernel, dim3(gx, (unsigned)nm), dim3(256), 0, s, os, ml);
+            EG_LAUNCH_CHECK();
         }
         res.num_groups = groups;
         return res;
