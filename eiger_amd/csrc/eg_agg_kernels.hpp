@@ -80,6 +80,28 @@ __device__ __forceinline__ void flush_lds(LdsTable<KB>& tab, const GTable& gt, c
     }
 }
 
+// Whether a fixed op list can keep per-lane partial states of one group in
+// registers: every op must merge associatively (no exchange, i.e. ANY).
+template <class OPS>
+__device__ constexpr bool privatisable()
+{
+    if constexpr (!OPS::kFixed) {
+        return false;
+    } else {
+        bool ok = true;
+        for (int a = 0; a < OPS::kN; ++a) ok = ok && op_class(OPS::word(a) & 15) != CLS_XCHG;
+        return ok;
+    }
+}
+
+// Ops held in registers by a privatising kernel (1 otherwise: unused).
+template <class OPS>
+__device__ constexpr int priv_ops()
+{
+    if constexpr (OPS::kFixed) return OPS::kN;
+    else return 1;
+}
+
 template <int KB, class VL, class OPS>
 __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const AggSpec& spec, const GTable& gt,
                                          int cr_log2, int rep_log2, int probe_limit, int peel_min)
@@ -102,6 +124,45 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
     const int r_mask = (1 << (use_lds ? rep_log2 : 0)) - 1;
     uint32_t ins = 0;  // HBM-table insertions by this lane
     dev::HotTags hot;  // groups the wave combined last
+    // The key held by at least peel_min lanes of a tile's first rows (lane
+    // 0's) accumulates in per-lane registers from then on; the wave's
+    // partials reach its slot (LDS replica 0, else the HBM table) with one
+    // wave reduction and one atomic per state when another key turns hot or
+    // the full tiles end.  Fixed op lists without ANY only.
+    using KT = typename LdsKey<KB>::T;
+    constexpr bool kPriv = privatisable<OPS>();
+    constexpr int kNP = kPriv ? priv_ops<OPS>() : 1;
+    uint64_t acc[kNP];
+#pragma unroll
+    for (int a = 0; a < kNP; ++a) acc[a] = 0ull;
+    uint64_t hot_key = ~0ull;  // wave-uniform
+    bool priv_on = false;      // wave-uniform: acc holds rows of hot_key
+    auto flush_priv = [&]() {
+        if constexpr (kPriv) {
+            if (!priv_on) return;
+            uint64_t tot[kNP];
+            dev::for_each_op<OPS>(spec, [&](auto a) { tot[a] = dev::wave_reduce(OPS::cls(spec, a), acc[a]); });
+            if (lane == 0) {
+                int sl = -1;
+                if (use_lds) {
+                    const KT e = (KB == 4) ? (KT)((uint32_t)hot_key ^ dev::kSign32)
+                                           : (KT)dev::enc64(dev::widen_key(hot_key, KB, key.is_signed));
+                    sl = tab.find(e, 0, probe_limit);
+                }
+                if (sl >= 0) {
+                    dev::for_each_op<OPS>(spec, [&](auto a) {
+                        dev::apply_atomic(OPS::cls(spec, a), &tab.st[(size_t)a * tab.T + sl], tot[a]);
+                    });
+                } else {
+                    unsigned long long* g = gt_slot(gt, dev::enc64(dev::widen_key(hot_key, KB, key.is_signed)), ins);
+                    if (g) dev::for_each_op<OPS>(spec, [&](auto a) { dev::apply_atomic(OPS::cls(spec, a), g + 1 + a, tot[a]); });
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < kNP; ++a) acc[a] = 0ull;
+            priv_on = false;
+        }
+    };
 
     // ---- full tiles: vector loads, next tile in flight while this one runs ----
     const int64_t tile = (int64_t)blockDim.x * kRows;
@@ -113,6 +174,16 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
         uint64_t knxt[kRows], vnxt[NV][kRows];
         const int64_t tn = t + gridDim.x;
         if (tn < full) load_tile<VL>(key, KB, spec, tn * tile + (int64_t)threadIdx.x * kRows, knxt, vnxt);
+        if constexpr (kPriv) {
+            if (peel_min <= 64) {
+                const uint64_t k0 = dev::readlane64(kcur[0], 0);
+                if (__popcll(__ballot(kcur[0] == k0)) >= (uint32_t)peel_min && !(priv_on && k0 == hot_key)) {
+                    flush_priv();
+                    hot_key = k0;
+                    priv_on = true;
+                }
+            }
+        }
         uint32_t spill = 0;
 #pragma unroll
         for (int i = 0; i < kRows; ++i) {
@@ -120,7 +191,16 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
 #pragma unroll
             for (int c = 0; c < NV; ++c) raw[c] = vcur[c][i];
             canon_row<VL, NV>(spec, raw, cv);
-            if (lds_row<OPS, KB, NV>(tab, use_lds, r_mask, lane, key, spec, kcur[i], cv, true, probe_limit, peel_min, hot))
+            bool mine = false;  // a row of the register-held hot key
+            if constexpr (kPriv) {
+                if (priv_on && kcur[i] == hot_key) {
+                    dev::for_each_op<OPS>(spec, [&](auto a) {
+                        acc[a] = dev::combine(OPS::cls(spec, a), acc[a], dev::op_operand<OPS, NV>(spec, a, cv));
+                    });
+                    mine = true;
+                }
+            }
+            if (lds_row<OPS, KB, NV>(tab, use_lds, r_mask, lane, key, spec, kcur[i], cv, !mine, probe_limit, peel_min, hot))
                 spill |= 1u << i;
         }
         if (__ballot(spill != 0) != 0ull) {
@@ -142,6 +222,7 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
             for (int c = 0; c < NV; ++c) vcur[c][i] = vnxt[c][i];
         }
     }
+    flush_priv();
     // ---- tail rows: one row per lane, typed loads ----
     for (int64_t b = full * tile + (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = b + threadIdx.x;
@@ -294,28 +375,6 @@ struct UnitTable {
         return -1;
     }
 };
-
-// Whether a fixed op list can keep per-lane partial states of one group in
-// registers: every op must merge associatively (no exchange, i.e. ANY).
-template <class OPS>
-__device__ constexpr bool privatisable()
-{
-    if constexpr (!OPS::kFixed) {
-        return false;
-    } else {
-        bool ok = true;
-        for (int a = 0; a < OPS::kN; ++a) ok = ok && op_class(OPS::word(a) & 15) != CLS_XCHG;
-        return ok;
-    }
-}
-
-// Ops held in registers by a privatising kernel (1 otherwise: unused).
-template <class OPS>
-__device__ constexpr int priv_ops()
-{
-    if constexpr (OPS::kFixed) return OPS::kN;
-    else return 1;
-}
 
 // GATHER: values are read through the row-id permutation `perm` (the
 // PARTITION_GFUR layout), else straight from the partitioned columns.
