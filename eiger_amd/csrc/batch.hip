@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kBatchBlock, 8) void batch_agg_kernel(const BQuery*
                 for (int c = 0; c < NV; ++c)
                     cv[c] = (valid && spec.nvcols > 0) ? dev::canon(dev::load_raw(svals, vbytes, r), spec.cols[0].conv) : 0ull;
                 const bool sp = lds_row<dev::RtOps, KB, NV>(tab, true, 0, lane, key, spec, kraw, cv, valid, probe_limit, peel_min, hot);
-                if (__ballot(sp) != 0ull) hbm_row<dev::RtOps, KB, NV>(gt, key, spec, sp, kraw, cv, 65, ins, hot);
+                if (__ballot(sp) != 0ull) hbm_rows<dev::RtOps, KB, NV>(gt, key, spec, sp, kraw, cv, 65, ins, hot);
             }
             __syncthreads();
             using KT = typename LdsKey<KB>::T;
@@ -384,7 +384,7 @@ __global__ __launch_bounds__(kBatchBlock, 8) void batch_agg_kernel(const BQuery*
                                                    : 0ull;
             bool sp = valid;
             if (use_lds) sp = lds_row<dev::RtOps, KB, NV>(tab, true, 0, lane, key, spec, kraw, cv, valid, probe_limit, peel_min, hot);
-            if (__ballot(sp) != 0ull) hbm_row<dev::RtOps, KB, NV>(gt, key, spec, sp, kraw, cv, use_lds ? 65 : peel_min, ins, hot);
+            if (__ballot(sp) != 0ull) hbm_rows<dev::RtOps, KB, NV>(gt, key, spec, sp, kraw, cv, use_lds ? 65 : peel_min, ins, hot);
         }
         if (use_lds) {
             __syncthreads();

@@ -212,7 +212,7 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
 #pragma unroll
                 for (int c = 0; c < NV; ++c) raw[c] = pick4(vcur[c], i);
                 canon_row<VL, NV>(spec, raw, cv);
-                hbm_row<OPS, KB, NV>(gt, key, spec, pend, pick4(kcur, i), cv, hbm_peel, ins, hot);
+                hbm_rows<OPS, KB, NV>(gt, key, spec, pend, pick4(kcur, i), cv, hbm_peel, ins, hot);
             }
         }
 #pragma unroll
@@ -234,7 +234,7 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
             raw[c] = (valid && c < spec.nvcols) ? dev::load_raw(spec.cols[c].data, spec.cols[c].bytes, r) : 0ull;
         canon_row<VL, NV>(spec, raw, cv);
         const bool sp = lds_row<OPS, KB, NV>(tab, use_lds, r_mask, lane, key, spec, kraw, cv, valid, probe_limit, peel_min, hot);
-        if (__ballot(sp) != 0ull) hbm_row<OPS, KB, NV>(gt, key, spec, sp, kraw, cv, hbm_peel, ins, hot);
+        if (__ballot(sp) != 0ull) hbm_rows<OPS, KB, NV>(gt, key, spec, sp, kraw, cv, hbm_peel, ins, hot);
     }
     if (use_lds) flush_lds<OPS>(tab, gt, key, spec, rep_log2, r_mask, ins);
     gt_count_inserts(gt, ins);

@@ -364,6 +364,94 @@ __device__ __forceinline__ void hbm_row(const GTable& gt, const KeyDesc& key, co
     }
 }
 
+// Fewest pending rows of a wave that hbm_rows redistributes over lane groups.
+constexpr int kGroupedMin = 16;
+
+// Index of the q-th set bit of m (0 <= q < popcount(m)).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int q)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __popcll((m >> pos) & ((1ull << w) - 1ull));
+        if (q >= c) {
+            q -= c;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+// Operand of op j (a per-lane index; 0 past the last op) for row values cv.
+template <class OPS, int NV>
+__device__ __forceinline__ uint64_t lane_operand(const AggSpec& spec, int j, const uint64_t (&cv)[NV])
+{
+    uint64_t val = 0ull;
+    dev::for_each_op<OPS>(spec, [&](auto a) {
+        const uint64_t x = dev::op_operand<OPS, NV>(spec, a, cv);
+        if ((int)a == j) val = x;
+    });
+    return val;
+}
+
+// The rows of lanes m (slot g, values cv) applied by groups of lanes, one row
+// a group and one op a lane.
+template <class OPS, int NV>
+__device__ __forceinline__ void hbm_grouped(const AggSpec& spec, uint64_t m, unsigned long long* g,
+                                                      const uint64_t (&cv)[NV])
+{
+    const int nops = OPS::n(spec);
+    const int lg = nops <= 8 ? 3 : 4;  // lanes per row
+    const int lane = dev::lane_id();
+    const int grp = lane >> lg, j = lane & ((1 << lg) - 1);
+    const int np = __popcll(m);
+    for (int base = 0; base < np; base += 64 >> lg) {
+        const int q = base + grp;
+        const int src = q < np ? nth_set_bit(m, q) : 0;
+        unsigned long long* d = (unsigned long long*)__shfl((unsigned long long)(uintptr_t)g, src);
+        uint64_t rv[NV];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) rv[c] = __shfl((unsigned long long)cv[c], src);
+        const uint64_t val = lane_operand<OPS, NV>(spec, j, rv);
+        if (q < np && j < nops) dev::apply_atomic(OPS::cls(spec, j), d + 1 + j, val);
+    }
+}
+
+// The wave's pending rows into the HBM table; every lane of the wave must be
+// at the call.  Each lane finds its own row's slot, then groups of 8 lanes
+// (16 for more than 8 ops) take one row each, lane j of a group applying op
+// j, so all of a row's states of one op class reach its slot's cache line as
+// a single request.  Device atomics on MI355X resolve at a fixed ~23 G
+// requests/s -- the same at agent or workgroup scope, with one table per XCD,
+// or with a table that fits L2 -- and a request may carry every lane that
+// hits its line: a row of five states costs three requests (one per class)
+// instead of five.  With wave combining on (peel_min <= 64) the rows of keys
+// shared by many lanes are combined first, as in hbm_row.
+template <class OPS, int KB, int NV>
+__device__ __forceinline__ void hbm_rows(const GTable& gt, const KeyDesc& key, const AggSpec& spec, bool pending,
+                                         uint64_t kraw, const uint64_t (&cv)[NV], int peel_min, uint32_t& ins,
+                                         dev::HotTags& hot)
+{
+    const uint64_t e64 = dev::enc64(dev::widen_key(kraw, KB, key.is_signed));
+    if (peel_min <= 64 && dev::wave_peel<OPS>(pending, e64, hot.hbm, spec, peel_min,
+                                              [&](int a) { return dev::op_operand<OPS, NV>(spec, a, cv); },
+                                              [&](uint64_t t) {
+                                                  unsigned long long* g = gt_slot(gt, t, ins);
+                                                  return g ? g + 1 : (unsigned long long*)nullptr;
+                                              },
+                                              (int64_t)1))
+        pending = false;
+    unsigned long long* g = nullptr;
+    if (pending) g = gt_slot(gt, e64, ins);
+    const uint64_t m = __ballot(g != nullptr);
+    if (__popcll(m) < kGroupedMin) {
+        // a few rows: the redistribution costs more instructions than the
+        // requests it saves
+        if (g) dev::apply_ops<OPS, NV>(spec, cv, [&](int a) { return g + 1 + a; });
+        return;
+    }
+    hbm_grouped<OPS, NV>(spec, m, g, cv);
+}
 
 }  // namespace lds
 }  // namespace eg
