@@ -89,18 +89,41 @@ constexpr int kFB = 1024;  // threads of the feature workgroup
 struct FeatLds {
     unsigned long long skey[kFSlots];
     uint32_t scnt[kFSlots];
-    float redf[kFB / 64];
+    float redf[4][kFB / 64];
     uint32_t redu[kFB / 64];
+    uint32_t top[kFB / 64][8];  // each wave's eight largest (count, slot) words
     uint32_t s_asc;
     unsigned long long s_min, s_max;
 };
+
+// The wave's eight largest of the values v[0..7] of its lanes (distinct
+// values), largest first, in every lane.
+__device__ __forceinline__ void wave_top8(uint32_t (&v)[8], uint32_t (&best)[8])
+{
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        uint32_t mine = 0u;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) mine = v[i] > mine ? v[i] : mine;
+        uint32_t m = mine;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const uint32_t o = __shfl_xor(m, off);
+            m = o > m ? o : m;
+        }
+        best[r] = m;
+        // the one lane holding m drops it (values are distinct or zero)
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (m != 0u && v[i] == m) v[i] = 0u;
+    }
+}
 
 // The 16 features of query q, written by thread 0 to o (kFB threads).
 __device__ void compute_features(const FQuery& q, FeatLds& L, float* o)
 {
     unsigned long long* skey = L.skey;
     uint32_t* scnt = L.scnt;
-    float* redf = L.redf;
     uint32_t* redu = L.redu;
     uint32_t& s_asc = L.s_asc;
     unsigned long long& s_min = L.s_min;
@@ -141,15 +164,32 @@ __device__ void compute_features(const FQuery& q, FeatLds& L, float* o)
             }
         }
     }
+    // span: a wave's minimum and maximum first, one LDS atomic each per wave
+    unsigned long long wmin = ~0ull, wmax = 0ull;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int i = t + j * kFB;
+        if (i >= ns) break;
+        const unsigned long long ek = ks[j] ^ 0x8000000000000000ull;  // order-preserving for the span
+        wmin = ek < wmin ? ek : wmin;
+        wmax = ek > wmax ? ek : wmax;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long a = __shfl_xor(wmin, off), b = __shfl_xor(wmax, off);
+        wmin = a < wmin ? a : wmin;
+        wmax = b > wmax ? b : wmax;
+    }
+    if ((t & 63) == 0 && wmax >= wmin) {
+        atomicMin(&s_min, wmin);
+        atomicMax(&s_max, wmax);
+    }
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int i = t + j * kFB;
         if (i >= ns) break;
         const uint64_t k = ks[j];
         if (i + 1 < ns) asc += ((int64_t)kn[j] >= (int64_t)k) ? 1u : 0u;
-        const uint64_t ek = k ^ 0x8000000000000000ull;  // order-preserving for the span
-        atomicMin(&s_min, (unsigned long long)ek);
-        atomicMax(&s_max, (unsigned long long)ek);
         const uint64_t tag = k + 1 == 0 ? 0x5bd1e9955bd1e995ull : k + 1;  // never 0
         uint32_t h = (uint32_t)(dev::fmix64(k) >> 40) & (kFSlots - 1);
         for (int p = 0; p < kFSlots; ++p) {
@@ -163,7 +203,9 @@ __device__ void compute_features(const FQuery& q, FeatLds& L, float* o)
         }
         atomicAdd(&scnt[h], 1u);
     }
-    atomicAdd(&s_asc, asc);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) asc += __shfl_xor(asc, off);
+    if ((t & 63) == 0 && asc != 0u) atomicAdd(&s_asc, asc);
     __syncthreads();
 
     // per-slot statistics
@@ -180,27 +222,68 @@ __device__ void compute_features(const FQuery& q, FeatLds& L, float* o)
         ent -= p * log2f(p);
         mx = c > mx ? c : mx;
     }
-    ndv = block_sum(ndv, redf);
-    f1 = block_sum(f1, redf);
-    f2 = block_sum(f2, redf);
-    ent = block_sum(ent, redf);
-    mx = block_max_u32(mx, redu);
-    // top-8 frequency mass: eight rounds of a block max over (count, slot)
-    // packed as count << 13 | (8191 - slot), removing the maximum each round
-    float top8 = 0.f;
-    for (int r = 0; r < 8; ++r) {
-        uint32_t m = 0;
-        for (int i = t; i < kFSlots; i += blockDim.x) {
-            const uint32_t c = scnt[i];
-            const uint32_t pk = c ? (c << 13) | (uint32_t)(kFSlots - 1 - i) : 0u;
-            m = pk > m ? pk : m;
+    // the four sums and the maximum in one exchange (the same shuffle tree
+    // and wave order as block_sum, so the same floats)
+    {
+        float f[4] = {ndv, f1, f2, ent};
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) f[a] += __shfl_xor(f[a], off);
+            const uint32_t o = __shfl_xor(mx, off);
+            mx = o > mx ? o : mx;
         }
-        m = block_max_u32(m, redu);
-        if (m == 0u) break;
-        top8 += (float)(m >> 13);
+        const int w = t >> 6, nw = blockDim.x >> 6;
+        if ((t & 63) == 0) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) L.redf[a][w] = f[a];
+            redu[w] = mx;
+        }
+        // top-8 frequency mass: each wave's eight largest (count, slot) words
+        // (count << 13 | (8191 - slot), distinct per slot), then the eight
+        // largest of those
+        constexpr int kPerT = kFSlots / kFB;
+        static_assert(kPerT == 8, "eight slots a thread");
+        uint32_t v[8], best[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int sl = t + i * kFB;
+            const uint32_t c = scnt[sl];
+            v[i] = c ? (c << 13) | (uint32_t)(kFSlots - 1 - sl) : 0u;
+        }
+        wave_top8(v, best);
+        if ((t & 63) == 0) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) L.top[w][r] = best[r];
+        }
         __syncthreads();
-        if (t == 0) scnt[kFSlots - 1 - (int)(m & (kFSlots - 1))] = 0u;
-        __syncthreads();
+        for (int a = 0; a < 4; ++a) {
+            float tot = 0.f;
+            for (int i = 0; i < nw; ++i) tot += L.redf[a][i];
+            f[a] = tot;
+        }
+        ndv = f[0];
+        f1 = f[1];
+        f2 = f[2];
+        ent = f[3];
+        uint32_t m = 0;
+        for (int i = 0; i < nw; ++i) m = redu[i] > m ? redu[i] : m;
+        mx = m;
+    }
+    if (t >= 64) return;
+    // wave 0: the eight largest of the waves' candidates (two per lane)
+    float top8 = 0.f;
+    {
+        const int nw = blockDim.x >> 6;
+        uint32_t v[8], best[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int c = t + i * 64;  // candidate c: wave c / 8, rank c % 8
+            v[i] = c < nw * 8 ? L.top[c >> 3][c & 7] : 0u;
+        }
+        wave_top8(v, best);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) top8 += (float)(best[r] >> 13);
     }
     if (t != 0) return;
     const float rows = (float)n;
@@ -241,24 +324,28 @@ __global__ __launch_bounds__(kFB) void select_kernel(FQuery q, const float* __re
 {
     __shared__ FeatLds L;
     __shared__ float x[kIn], h1[kH], h2[kH];
+    __shared__ float W[kParams];
+    // the weights reach LDS while the sample is read (every thread's loads
+    // are issued before compute_features' first barrier)
+    for (int i = threadIdx.x; i < kParams; i += kFB) W[i] = P[i];
     compute_features(q, L, x);
     __syncthreads();
     const int u = threadIdx.x;
     if (u < kH) {
-        float z = P[kB1 + u];
-        for (int f = 0; f < kIn; ++f) z = fmaf(P[kW1 + u * kIn + f], x[f], z);
+        float z = W[kB1 + u];
+        for (int f = 0; f < kIn; ++f) z = fmaf(W[kW1 + u * kIn + f], x[f], z);
         h1[u] = z > 0.f ? z : 0.f;
     }
     __syncthreads();
     if (u < kH) {
-        float z = P[kB2 + u];
-        for (int v = 0; v < kH; ++v) z = fmaf(P[kW2 + u * kH + v], h1[v], z);
+        float z = W[kB2 + u];
+        for (int v = 0; v < kH; ++v) z = fmaf(W[kW2 + u * kH + v], h1[v], z);
         h2[u] = z > 0.f ? z : 0.f;
     }
     __syncthreads();
     if (u < kOut) {
-        float z = P[kB3 + u];
-        for (int v = 0; v < kH; ++v) z = fmaf(P[kW3 + u * kH + v], h2[v], z);
+        float z = W[kB3 + u];
+        for (int v = 0; v < kH; ++v) z = fmaf(W[kW3 + u * kH + v], h2[v], z);
         y[u] = z;
     }
     if (u == kOut) y[kOut] = x[7];  // the sample's singleton fraction
