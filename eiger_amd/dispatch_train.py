@@ -2,6 +2,8 @@
 
     python -m eiger_amd.dispatch_train --train 50000 --heldout 10000 --out DIR
 
+(or in parts: --trace-part train:0:25000 ... then --from-traces)
+
 1. Queries are drawn like the two benchmark workloads: (a) batched uneven
    queries -- rows log-uniform in [1e3, 4e6], distinct count log-uniform in
    [1, rows/2], uniform or Zipf s ~ U[0.5, 1.5], SUM(int64) + COUNT; (b)
@@ -12,8 +14,9 @@
    kernels specialised for each request list are compiled before timing, as
    a long-running caller would have them.
 2. Each query's 16 sample features come from the HIP feature kernel; each
-   strategy (hash, sort, partition) is timed on it, with the true group count
-   as the hint (the label is what each strategy costs when sized right).
+   strategy (hash, sort, partition) is timed on it with the group bound the
+   learned dispatcher would size it with (from the true group count), so the
+   label is what each strategy costs when the dispatcher picks it.
 3. The MLP is trained on the device by eg_mlp_train (HIP forward/backward +
    Adam) against soft targets p_i ~ exp(-(t_i / t_best - 1) / tau) and the
    log2 group count.
@@ -116,11 +119,22 @@ def trace(specs, seed0: int, wl: Workload, progress_every: int = 500):
         h = G.groupby_hint()
         _, rule_ms[i] = timed(G.auto_select_groupby_algo, [k], h)
         rule[i] = ALGO_INDEX.get(h.groupby_algo, 0)
+        # the true group count (untimed), then every strategy sized the way the
+        # learned dispatcher sizes the one it picks (api.hip: 2.5x the
+        # estimate, or the HyperLogLog estimate + 25% when the sample is mostly
+        # singletons), with the true count standing in for the estimates
+        t = FNS["hash"]([k], vals, reqs, G.groupby_hint(groups_upper_bound=spec[2]))
+        true = int(t.num_rows())
+        t.free()
+        rows = spec[1]
+        caller = spec[2] if 0 < spec[2] < rows else rows
+        bound = min(caller, 2 * true + true // 2 + 1024)
+        if bound < caller and feats[i][7] > 0.25:
+            bound = min(caller, true + true // 4 + 1024)
+        groups[i] = true
         for j, name in enumerate(STRATS):
-            hint = G.groupby_hint(groups_upper_bound=spec[2])
+            hint = G.groupby_hint(groups_upper_bound=bound)
             t, times[i, j] = timed(FNS[name], [k], vals, reqs, hint)
-            if j == 0:
-                groups[i] = t.num_rows()
             t.free()
         k.free()
         if (i + 1) % progress_every == 0:
@@ -214,26 +228,56 @@ def main():
     p.add_argument("--heldout", type=int, default=10000)
     p.add_argument("--epochs", type=int, default=80)
     p.add_argument("--out", default="dispatch_out")
+    p.add_argument("--trace-part", default=None,
+                   help="SET:A:B (SET train or heldout): trace queries [A, B) of SET into "
+                        "OUT/trace_SET_A_B.npz and stop, so a long trace can span several runs")
+    p.add_argument("--from-traces", action="store_true",
+                   help="train and evaluate from the OUT/trace_*.npz parts instead of tracing")
     args = p.parse_args()
     os.makedirs(args.out, exist_ok=True)
 
     specs_tr = draw_queries(args.train, seed=1)
     specs_ho = draw_queries(args.heldout, seed=2)
+    seeds = {"train": 1_000_000, "heldout": 2_000_000}
     wl = Workload(max(s[1] for s in specs_tr + specs_ho))
     G.jit_configure(G.jit_mode.SYNC, None)  # specialised kernels compile on first use, before timing
-    # warm every strategy once, on both request lists at sizes that use the
-    # specialised kernels
-    warm = [("batch", 1 << 21, 1000, 0.0), ("sweep", 1 << 21, 1000, 0.0), ("sweep", 1 << 21, 1 << 20, 0.0)]
-    for spec in warm + specs_tr[:8]:
-        k, vals, reqs, _, _ = wl.query(spec, 0)
-        for name in STRATS:
-            FNS[name]([k], vals, reqs, G.groupby_hint(groups_upper_bound=spec[2])).free()
-        k.free()
+    if not args.from_traces:
+        # warm every strategy once, on both request lists at sizes that use the
+        # specialised kernels
+        warm = [("batch", 1 << 21, 1000, 0.0), ("sweep", 1 << 21, 1000, 0.0), ("sweep", 1 << 21, 1 << 20, 0.0)]
+        for spec in warm + specs_tr[:8]:
+            k, vals, reqs, _, _ = wl.query(spec, 0)
+            for name in STRATS:
+                FNS[name]([k], vals, reqs, G.groupby_hint(groups_upper_bound=spec[2])).free()
+            k.free()
     G.mlp_load(G.mlp_init(0))  # any weights: the trace times the dispatcher's select path
-    log("tracing training queries")
-    tr = trace(specs_tr, 1_000_000, wl)
-    log("tracing held-out queries")
-    ho = trace(specs_ho, 2_000_000, wl)
+    if args.trace_part:
+        name, a, b = args.trace_part.split(":")
+        a, b = int(a), int(b)
+        specs = specs_tr if name == "train" else specs_ho
+        log(f"tracing {name} queries [{a}, {b})")
+        part = trace(specs[a:b], seeds[name] + a, wl)
+        np.savez_compressed(os.path.join(args.out, f"trace_{name}_{a}_{b}.npz"), **part)
+        return
+    if args.from_traces:
+        def load(name, n):
+            parts = []
+            for f in os.listdir(args.out):
+                if f.startswith(f"trace_{name}_") and f.endswith(".npz"):
+                    a, b = (int(x) for x in f[len(f"trace_{name}_"):-4].split("_"))
+                    parts.append((a, b, f))
+            parts.sort()
+            if not parts or parts[0][0] != 0 or parts[-1][1] != n or any(
+                    parts[i][1] != parts[i + 1][0] for i in range(len(parts) - 1)):
+                raise SystemExit(f"trace parts of {name} do not cover [0, {n}): {[(a, b) for a, b, _ in parts]}")
+            data = [np.load(os.path.join(args.out, f)) for _, _, f in parts]
+            return {k: np.concatenate([d[k] for d in data]) for k in data[0].files}
+        tr, ho = load("train", len(specs_tr)), load("heldout", len(specs_ho))
+    else:
+        log("tracing training queries")
+        tr = trace(specs_tr, seeds["train"], wl)
+        log("tracing held-out queries")
+        ho = trace(specs_ho, seeds["heldout"], wl)
 
     card = (np.log2(np.maximum(tr["groups"], 1)) / 32).astype(np.float32)
     # model selection on a validation split of the training trace: the soft
