@@ -83,18 +83,41 @@ int hll_parts(int64_t n)
     return (int)std::max<int64_t>(1, std::min<int64_t>((n + 16383) / 16384, (int64_t)device_info().num_cus * 2));
 }
 
-__global__ void hll_merge_kernel(const uint8_t* __restrict__ partial, int parts, uint8_t* __restrict__ out)
+// Per-register maximum over the partial register sets.  With acc set, also
+// the estimator's inputs: acc[0] += sum of 2^(40 - r) (the harmonic sum in
+// units of 2^-40: exact, as the host's double sum is, while r <= 39, i.e.
+// for any column below ~2^50 distinct keys) and acc[1] += the zero registers.
+__global__ __launch_bounds__(256) void hll_merge_kernel(const uint8_t* __restrict__ partial, int parts,
+                                                        uint8_t* __restrict__ out, unsigned long long* acc)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kM) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // the grid covers kM exactly
     uint8_t m = 0;
     for (int p = 0; p < parts; ++p) m = std::max(m, partial[(size_t)p * kM + i]);
     out[i] = m;
+    if (!acc) return;
+    unsigned long long sum = m <= 40 ? 1ull << (40 - m) : 0ull, zeros = m == 0 ? 1ull : 0ull;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        sum += __shfl_xor(sum, off);
+        zeros += __shfl_xor(zeros, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&acc[0], sum);
+        atomicAdd(&acc[1], zeros);
+    }
 }
 
 }  // namespace
 
 namespace {
+double hll_estimate_from(double sum, int64_t zeros)
+{
+    const double alpha = 0.7213 / (1.0 + 1.079 / kM);
+    double e = alpha * (double)kM * (double)kM / sum;
+    if (e <= 2.5 * kM && zeros > 0) e = (double)kM * std::log((double)kM / (double)zeros);
+    return e;
+}
+
 double hll_estimate(const uint8_t* reg)
 {
     double sum = 0.0;
@@ -117,7 +140,7 @@ void hll_registers(const void* data, int bytes, int64_t n, uint8_t* regs, DevBuf
     KernelTimer timer("hll", s);
     launch_hll(data, bytes, n, parts, partial.get<uint8_t>(), s);
     hipLaunchKernelGGL(hll_merge_kernel, dim3(kM / 256), dim3(256), 0, s, (const uint8_t*)partial.get<uint8_t>(), parts,
-                       regs);
+                       regs, (unsigned long long*)nullptr);
     EG_LAUNCH_CHECK();
 }
 }  // namespace
@@ -128,18 +151,24 @@ int64_t ndv_approx(const void* data, int32_t type, int64_t n, hipStream_t s)
     if (!bytes) raise(EG_ERR_UNSUPPORTED, "compute_ndv_approx: unsupported column type");
     if (n <= 0) return 0;
     const int parts = hll_parts(n);
-    DevBuf partial((size_t)parts * kM + kM, s);
+    // partial registers, the merged registers, then the two estimator sums
+    DevBuf partial((size_t)parts * kM + kM + 16, s);
     uint8_t* pp = partial.get<uint8_t>();
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(pp + (size_t)parts * kM + kM);
+    EG_HIP(hipMemsetAsync(acc, 0, 16, s));
     {
         KernelTimer timer("hll", s);
         launch_hll(data, bytes, n, parts, pp, s);
-        hipLaunchKernelGGL(hll_merge_kernel, dim3(kM / 256), dim3(256), 0, s, pp, parts, pp + (size_t)parts * kM);
+        hipLaunchKernelGGL(hll_merge_kernel, dim3(kM / 256), dim3(256), 0, s, (const uint8_t*)pp, parts,
+                           pp + (size_t)parts * kM, acc);
         EG_LAUNCH_CHECK();
     }
-    std::vector<uint8_t> reg(kM);
-    EG_HIP(hipMemcpyAsync(reg.data(), pp + (size_t)parts * kM, kM, hipMemcpyDeviceToHost, s));
+    // 16 bytes back instead of the 16 KB of registers
+    int64_t* h = pinned_i64(2);
+    EG_HIP(hipMemcpyAsync(h, acc, 16, hipMemcpyDeviceToHost, s));
     EG_HIP(hipStreamSynchronize(s));
-    int64_t r = (int64_t)std::llround(hll_estimate(reg.data()));
+    const double sum = std::ldexp((double)(uint64_t)h[0], -40);
+    int64_t r = (int64_t)std::llround(hll_estimate_from(sum, h[1]));
     return std::max<int64_t>(1, std::min<int64_t>(r, n));
 }
 
