@@ -145,8 +145,10 @@ private:
     hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 
-// Wall time of a host-orchestrated phase (synchronises the stream at both
-// ends, only while timing is enabled); recorded under `name`.
+// Wall time of a host-orchestrated phase, recorded under `name`.  It
+// synchronises the stream at both ends, so it is on only while timing is
+// enabled and EG_TIME_HOST_PHASES=1 (a diagnostic: with it, kernel timing
+// would no longer time the library as it runs).
 class HostPhase {
 public:
     HostPhase(const char* name, hipStream_t s);

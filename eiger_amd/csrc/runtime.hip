@@ -350,15 +350,31 @@ struct TimingRec {
 std::mutex g_t_mu;
 bool g_timing = false;
 std::vector<TimingRec> g_recs;
+std::vector<hipEvent_t> g_event_pool;
 }  // namespace
 
 bool timing_enabled() { return g_timing; }
 
+// Events come from a pool refilled by timing_reset, so a timed launch costs
+// two event records and no event creation.
+static hipEvent_t pooled_event()
+{
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    EG_HIP(hipEventCreate(&e));
+    return e;
+}
+
 KernelTimer::KernelTimer(const char* name, hipStream_t s) : name_(name), s_(s)
 {
     if (!g_timing) return;
-    EG_HIP(hipEventCreate(&a_));
-    EG_HIP(hipEventCreate(&b_));
+    a_ = pooled_event();
+    b_ = pooled_event();
     EG_HIP(hipEventRecord(a_, s_));
 }
 
@@ -372,7 +388,11 @@ KernelTimer::~KernelTimer()
 
 HostPhase::HostPhase(const char* name, hipStream_t s) : name_(name), s_(s)
 {
-    if (!g_timing) return;
+    static const bool enabled = [] {
+        const char* e = std::getenv("EG_TIME_HOST_PHASES");
+        return e && std::atoi(e) == 1;
+    }();
+    if (!g_timing || !enabled) return;
     on_ = true;
     (void)hipStreamSynchronize(s_);
     t0_ = std::chrono::steady_clock::now();
@@ -396,6 +416,15 @@ void timing_count(const char* name)
 
 int timing_enable(int on)
 {
+    if (on) {
+        // events for the first ~1000 timed launches, created before timing starts
+        std::lock_guard<std::mutex> lk(g_t_mu);
+        while (g_event_pool.size() < 2048) {
+            hipEvent_t e = nullptr;
+            EG_HIP(hipEventCreate(&e));
+            g_event_pool.push_back(e);
+        }
+    }
     g_timing = on != 0;
     return EG_OK;
 }
@@ -405,8 +434,14 @@ int timing_reset()
     std::lock_guard<std::mutex> lk(g_t_mu);
     for (auto& r : g_recs) {
         if (!r.a) continue;  // host phase: no events
-        (void)hipEventDestroy(r.a);
-        (void)hipEventDestroy(r.b);
+        // back to the pool once recorded work has passed them
+        if (hipEventSynchronize(r.b) == hipSuccess) {
+            g_event_pool.push_back(r.a);
+            g_event_pool.push_back(r.b);
+        } else {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
     }
     g_recs.clear();
     return EG_OK;
