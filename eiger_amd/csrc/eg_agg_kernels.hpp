@@ -204,15 +204,42 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
                 spill |= 1u << i;
         }
         if (__ballot(spill != 0) != 0ull) {
+            if (hbm_peel <= 64) {
 #pragma unroll 1
-            for (int i = 0; i < kRows; ++i) {
-                const bool pend = (spill >> i) & 1u;
-                if (__ballot(pend) == 0ull) continue;
-                uint64_t raw[NV], cv[NV];
+                for (int i = 0; i < kRows; ++i) {
+                    const bool pend = (spill >> i) & 1u;
+                    if (__ballot(pend) == 0ull) continue;
+                    uint64_t raw[NV], cv[NV];
 #pragma unroll
-                for (int c = 0; c < NV; ++c) raw[c] = pick4(vcur[c], i);
-                canon_row<VL, NV>(spec, raw, cv);
-                hbm_rows<OPS, KB, NV>(gt, key, spec, pend, pick4(kcur, i), cv, hbm_peel, ins, hot);
+                    for (int c = 0; c < NV; ++c) raw[c] = pick4(vcur[c], i);
+                    canon_row<VL, NV>(spec, raw, cv);
+                    hbm_rows<OPS, KB, NV>(gt, key, spec, pend, pick4(kcur, i), cv, hbm_peel, ins, hot);
+                }
+            } else {
+                // every spilled row of the step finds its slot, then all of
+                // them share the lane-group rounds (one request per op class
+                // and row); a few rows go one per lane
+                unsigned long long* g[kRows];
+                uint64_t cvs[kRows][NV], m[kRows];
+                int np = 0;
+#pragma unroll
+                for (int i = 0; i < kRows; ++i) {
+                    uint64_t raw[NV];
+#pragma unroll
+                    for (int c = 0; c < NV; ++c) raw[c] = vcur[c][i];
+                    canon_row<VL, NV>(spec, raw, cvs[i]);
+                    g[i] = nullptr;
+                    if ((spill >> i) & 1u) g[i] = gt_slot(gt, dev::enc64(dev::widen_key(kcur[i], KB, key.is_signed)), ins);
+                    m[i] = __ballot(g[i] != nullptr);
+                    np += __popcll(m[i]);
+                }
+                if (np >= kGroupedMinStep) {
+                    hbm_grouped_multi<OPS, NV, kRows>(spec, m, g, cvs);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < kRows; ++i)
+                        if (g[i]) dev::apply_ops<OPS, NV>(spec, cvs[i], [&](int a) { return g[i] + 1 + a; });
+                }
             }
         }
 #pragma unroll

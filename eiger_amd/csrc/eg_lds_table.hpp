@@ -364,8 +364,11 @@ __device__ __forceinline__ void hbm_row(const GTable& gt, const KeyDesc& key, co
     }
 }
 
-// Fewest pending rows of a wave that hbm_rows redistributes over lane groups.
+// Fewest pending rows of a wave that hbm_rows redistributes over lane groups
+// (one row per lane), and that a hash-kernel step does (four rows per lane,
+// whose rows share the rounds).
 constexpr int kGroupedMin = 16;
+constexpr int kGroupedMinStep = 6;
 
 // Index of the q-th set bit of m (0 <= q < popcount(m)).
 __device__ __forceinline__ int nth_set_bit(uint64_t m, int q)
@@ -412,6 +415,54 @@ __device__ __forceinline__ void hbm_grouped(const AggSpec& spec, uint64_t m, uns
         uint64_t rv[NV];
 #pragma unroll
         for (int c = 0; c < NV; ++c) rv[c] = __shfl((unsigned long long)cv[c], src);
+        const uint64_t val = lane_operand<OPS, NV>(spec, j, rv);
+        if (q < np && j < nops) dev::apply_atomic(OPS::cls(spec, j), d + 1 + j, val);
+    }
+}
+
+// Like hbm_grouped for rows spread over R registers per lane: row k of lane l
+// is pending when bit l of m[k] is set, with slot g[k] and values cv[k].
+// The rows of all R sets share the rounds, so a wave whose lanes each spill
+// a row or two still fills its groups.
+template <class OPS, int NV, int R>
+__device__ __forceinline__ void hbm_grouped_multi(const AggSpec& spec, const uint64_t (&m)[R],
+                                                  unsigned long long* const (&g)[R], const uint64_t (&cv)[R][NV])
+{
+    const int nops = OPS::n(spec);
+    const int lg = nops <= 8 ? 3 : 4;  // lanes per row
+    const int lane = dev::lane_id();
+    const int grp = lane >> lg, j = lane & ((1 << lg) - 1);
+    int pre[R + 1];  // wave-uniform prefix counts
+    pre[0] = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) pre[k + 1] = pre[k] + __popcll(m[k]);
+    const int np = pre[R];
+    for (int base = 0; base < np; base += 64 >> lg) {
+        const int q = base + grp;
+        int r = 0, qr = q;
+        uint64_t mr = m[0];
+#pragma unroll
+        for (int k = 1; k < R; ++k)
+            if (q >= pre[k]) {
+                r = k;
+                mr = m[k];
+                qr = q - pre[k];
+            }
+        const int src = q < np ? nth_set_bit(mr, qr) : 0;
+        unsigned long long* d = nullptr;
+        uint64_t rv[NV];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) rv[c] = 0ull;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            unsigned long long* dk = (unsigned long long*)__shfl((unsigned long long)(uintptr_t)g[k], src);
+            if (r == k) d = dk;
+#pragma unroll
+            for (int c = 0; c < NV; ++c) {
+                const uint64_t x = __shfl((unsigned long long)cv[k][c], src);
+                if (r == k) rv[c] = x;
+            }
+        }
         const uint64_t val = lane_operand<OPS, NV>(spec, j, rv);
         if (q < np && j < nops) dev::apply_atomic(OPS::cls(spec, j), d + 1 + j, val);
     }
