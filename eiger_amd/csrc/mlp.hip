@@ -606,6 +606,14 @@ int algo_of(const float* o, int64_t rows)
     return best == 0 ? EG_ALGO_HASH : best == 1 ? EG_ALGO_SORT : EG_ALGO_PARTITION;
 }
 
+// A large input whose caller-given group bound already fits one LDS table
+// goes to hash without sampling: the bound caps the true group count, so
+// algo_of's rule would pick hash for any estimate the model could give.
+bool bound_picks_hash(int64_t rows, int64_t group_bound)
+{
+    return rows >= ((int64_t)1 << 24) && group_bound > 0 && group_bound <= 2048;
+}
+
 // Learned strategy choice for one key column; false when no model is loaded.
 bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_bound, int nops, int value_bytes,
                 hipStream_t s, int32_t* algo, int64_t* card_est, float* singletons)
@@ -613,10 +621,10 @@ bool mlp_select(const void* keys, int32_t key_type, int64_t rows, int64_t group_
     LoadedModel& m = loaded_model();
     std::lock_guard<std::mutex> lk(m.mu);
     if (!m.loaded) return false;
-    if (rows < kSmallRows) {
+    if (rows < kSmallRows || bound_picks_hash(rows, group_bound)) {
         // hash, sized by the caller's bound (or the row count)
         *algo = EG_ALGO_HASH;
-        if (card_est) *card_est = rows;
+        if (card_est) *card_est = rows < kSmallRows ? rows : group_bound;
         if (singletons) *singletons = 0.f;
         return true;
     }
@@ -832,11 +840,13 @@ int eg_mlp_select_batch(const eg_mlp_query* queries, int64_t num, int32_t* algos
         EG_HIP(hipStreamSynchronize(s));
         for (int64_t i = 0; i < num; ++i) {
             const float* o = y.data() + (size_t)i * eg::kOut;
-            algos[i] = eg::algo_of(o, queries[i].rows);
+            const bool by_bound = eg::bound_picks_hash(queries[i].rows, queries[i].group_bound);
+            algos[i] = by_bound ? EG_ALGO_HASH : eg::algo_of(o, queries[i].rows);
             if (card_est) {
                 const double lg = std::min(62.0, std::max(0.0, (double)o[3] * 32.0));
                 card_est[i] = queries[i].rows < eg::kSmallRows  // as mlp_select: hash sized by the row count
                                   ? queries[i].rows
+                              : by_bound ? queries[i].group_bound
                                   : (int64_t)std::min<double>((double)std::max<int64_t>(queries[i].rows, 1), std::exp2(lg));
             }
         }

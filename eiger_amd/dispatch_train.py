@@ -151,11 +151,13 @@ def soft_targets(times: np.ndarray, tau: float) -> np.ndarray:
 def picks(y: np.ndarray, features: np.ndarray) -> np.ndarray:
     """Strategy index (hash, sort, partition) the library picks from model
     outputs y: the best class score, except that inputs of >= 2^24 rows whose
-    estimated group count is <= 2048 go to hash (mlp.hip algo_of)."""
+    estimated group count, or group bound, is <= 2048 go to hash (mlp.hip
+    algo_of, bound_picks_hash)."""
     pick = np.argmax(y[:, :3], axis=1)
     rows = np.exp2(features[:, 0].astype(np.float64) * 32) - 1
     card = np.exp2(np.clip(y[:, 3].astype(np.float64) * 32, 0, 62))
-    return np.where((rows >= 2.0 ** 24) & (card <= 2048), 0, pick)
+    bound = np.exp2(features[:, 11].astype(np.float64) * 32) - 1
+    return np.where((rows >= 2.0 ** 24) & ((card <= 2048) | (bound <= 2048.5)), 0, pick)
 
 
 def choice_ms(params: np.ndarray, tr: dict) -> float:
