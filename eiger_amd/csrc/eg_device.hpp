@@ -450,5 +450,34 @@ __device__ __forceinline__ int64_t block_claim(unsigned long long* counter, bool
     return r;
 }
 
+// Like block_claim for n (0..) rows per thread: this thread's first index,
+// the block's rows contiguous in thread order.
+__device__ __forceinline__ int64_t block_claim_n(unsigned long long* counter, uint32_t n, unsigned long long* sh)
+{
+    uint32_t x = n;  // inclusive scan over the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane_id() >= off) x += y;
+    }
+    const int w = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    if (lane_id() == 63) sh[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int i = 0; i < nw; ++i) {
+            const unsigned long long c = sh[i];
+            sh[i] = tot;
+            tot += c;
+        }
+        sh[16] = tot ? atomicAdd(counter, tot) : 0ull;
+    }
+    __syncthreads();
+    const int64_t r = (int64_t)(sh[16] + sh[w] + (unsigned long long)(x - n));
+    __syncthreads();  // sh is reused by the next call
+    return r;
+}
+
 }  // namespace dev
 }  // namespace eg

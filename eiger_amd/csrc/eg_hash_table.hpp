@@ -60,6 +60,43 @@ __device__ __forceinline__ unsigned long long* gt_slot(const GTable& gt, uint64_
     return nullptr;
 }
 
+// Every group of the table (slots, then the side slot) handed to
+// emit(index, enc, states), indices dense from *counter on.  A thread looks
+// at kCompactSlots slots per round and a block claims its round's groups with
+// one device atomic: same-address atomics serialise (~80 per microsecond), so
+// one claim per 256 slots cost a 3M-slot table ~0.2 ms.
+constexpr int kCompactSlots = 8;
+template <class EMIT>
+__device__ __forceinline__ void compact_groups(const GTable& gt, unsigned long long* counter, unsigned long long* sh,
+                                               EMIT emit)
+{
+    const uint64_t total = gt.mask + 2;  // slots + side slot
+    const uint64_t chunk = (uint64_t)blockDim.x * kCompactSlots;
+    for (uint64_t base = (uint64_t)blockIdx.x * chunk; base < total; base += (uint64_t)gridDim.x * chunk) {
+        uint64_t enc[kCompactSlots];
+        uint32_t occ = 0;
+#pragma unroll
+        for (int i = 0; i < kCompactSlots; ++i) {
+            const uint64_t s = base + (uint64_t)i * blockDim.x + threadIdx.x;
+            enc[i] = 0;
+            if (s <= gt.mask) {
+                enc[i] = gt.base[s * (uint64_t)gt.W];
+                if (enc[i] != 0) occ |= 1u << i;
+            } else if (s == gt.mask + 1 && gt.meta[1] != 0) {
+                occ |= 1u << i;
+            }
+        }
+        int64_t idx = dev::block_claim_n(counter, (uint32_t)__popc(occ), sh);
+#pragma unroll
+        for (int i = 0; i < kCompactSlots; ++i) {
+            if (!((occ >> i) & 1u)) continue;
+            const uint64_t s = base + (uint64_t)i * blockDim.x + threadIdx.x;
+            const unsigned long long* slot = s <= gt.mask ? gt.base + s * (uint64_t)gt.W : gt.side;
+            emit(idx++, enc[i], slot + 1);
+        }
+    }
+}
+
 // Adds the wave's insertion count to meta[0] with one atomic.  Every lane of
 // the wave calls it (at the end of a kernel).
 __device__ __forceinline__ void gt_count_inserts(const GTable& gt, uint32_t ins)

@@ -52,28 +52,12 @@ __global__ __launch_bounds__(max_block<VL::NV>()) void hash_agg_kernel(KeyDesc k
     agg::hash_agg<KB, VL, dev::RtOps>(key, n, spec, gt, cr_log2, rep_log2, probe_limit, peel_min);
 }
 
-__global__ void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter)
+__global__ __launch_bounds__(256) void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter)
 {
     __shared__ unsigned long long sh[17];
-    const uint64_t total = gt.mask + 2;  // slots + side slot
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
-        uint64_t s = base + threadIdx.x;
-        bool occ = false;
-        const unsigned long long* slot = nullptr;
-        uint64_t enc = 0;
-        if (s <= gt.mask) {
-            slot = gt.base + s * (uint64_t)gt.W;
-            enc = slot[0];
-            occ = enc != 0;
-        } else if (s == gt.mask + 1) {
-            slot = gt.side;
-            enc = 0;
-            occ = gt.meta[1] != 0;
-        }
-        const int64_t idx = dev::block_claim(counter, occ, sh);
-        if (occ) dev::emit_group(os, idx, enc, slot + 1, 1);
-    }
+    compact_groups(gt, counter, sh, [&](int64_t idx, uint64_t enc, const unsigned long long* st) {
+        dev::emit_group(os, idx, enc, st, 1);
+    });
 }
 
 // use_jit: take the specialised kernel once it is loaded (loading a module
@@ -291,8 +275,9 @@ int64_t run_hash_noalloc(const Plan& plan, int64_t group_bound, const OutSpec& o
                                        std::to_string(capacity) + " rows");
         if (groups > 0) {
             const uint64_t total = gt.mask + 2;
+            const int64_t per = 256 * kCompactSlots;
             const int64_t grid =
-                std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + 255) / 256), (int64_t)di.num_cus * 8));
+                std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + per - 1) / per), (int64_t)di.num_cus * 8));
             KernelTimer timer("hash_compact", s);
             hipLaunchKernelGGL(hash_compact_kernel, dim3((unsigned)grid), dim3(256), 0, s, gt, os, gt.meta + 3);
             EG_LAUNCH_CHECK();
@@ -336,7 +321,8 @@ void HbmTable::compact(const OutSpec& os, unsigned long long* counter, hipStream
 {
     const DeviceInfo& di = device_info();
     const uint64_t total = gt.mask + 2;
-    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + 255) / 256), (int64_t)di.num_cus * 8));
+    const int64_t per = 256 * kCompactSlots;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + per - 1) / per), (int64_t)di.num_cus * 8));
     KernelTimer timer("hash_compact", s);
     hipLaunchKernelGGL(hash_compact_kernel, dim3((unsigned)grid), dim3(256), 0, s, gt, os, counter);
     EG_LAUNCH_CHECK();

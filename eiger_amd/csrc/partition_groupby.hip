@@ -122,27 +122,13 @@ __global__ void part_bounds_kernel(const void* keys, int64_t n, uint32_t mask, i
 }
 
 // Appends the HBM table's groups after the directly written ones.
-__global__ void part_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter, int64_t capacity)
+__global__ __launch_bounds__(256) void part_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter,
+                                                           int64_t capacity)
 {
     __shared__ unsigned long long sh[17];
-    const uint64_t total = gt.mask + 2;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
-        const uint64_t s = base + threadIdx.x;
-        bool occ = false;
-        const unsigned long long* slot = nullptr;
-        uint64_t enc = 0;
-        if (s <= gt.mask) {
-            slot = gt.base + s * (uint64_t)gt.W;
-            enc = slot[0];
-            occ = enc != 0;
-        } else if (s == gt.mask + 1) {
-            slot = gt.side;
-            occ = gt.meta[1] != 0;
-        }
-        const int64_t idx = dev::block_claim(counter, occ, sh);
-        if (occ && idx < capacity) dev::emit_group(os, idx, enc, slot + 1, 1);
-    }
+    compact_groups(gt, counter, sh, [&](int64_t idx, uint64_t enc, const unsigned long long* st) {
+        if (idx < capacity) dev::emit_group(os, idx, enc, st, 1);
+    });
 }
 
 // Output rows past the group count moved into the holes below it (the
@@ -401,7 +387,8 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
         if (!table_full) {
             // append HBM-table groups after every claimed chunk
             const uint64_t total = ht.gt.mask + 2;
-            const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + 255) / 256), (int64_t)di.num_cus * 8));
+            const int64_t per = 256 * kCompactSlots;
+            const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + per - 1) / per), (int64_t)di.num_cus * 8));
             KernelTimer timer("partition_compact", s);
             hipLaunchKernelGGL(part_compact_kernel, dim3((unsigned)grid), dim3(256), 0, s, ht.gt, os,
                                claims.words, cap_rows);
