@@ -102,6 +102,10 @@ __device__ constexpr int priv_ops()
     else return 1;
 }
 
+// Fewest of a step's first rows (one per lane) holding lane 0's key for the
+// hash kernel to keep that key in registers.
+constexpr int kPrivLanes = 8;
+
 template <int KB, class VL, class OPS>
 __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const AggSpec& spec, const GTable& gt,
                                          int cr_log2, int rep_log2, int probe_limit, int peel_min)
@@ -124,8 +128,8 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
     const int r_mask = (1 << (use_lds ? rep_log2 : 0)) - 1;
     uint32_t ins = 0;  // HBM-table insertions by this lane
     dev::HotTags hot;  // groups the wave combined last
-    // The key held by at least peel_min lanes of a tile's first rows (lane
-    // 0's) accumulates in per-lane registers from then on; the wave's
+    // A key held by many lanes of a tile's first rows (lane 0's, see
+    // kPrivLanes) accumulates in per-lane registers from then on; the wave's
     // partials reach its slot (LDS replica 0, else the HBM table) with one
     // wave reduction and one atomic per state when another key turns hot or
     // the full tiles end.  Fixed op lists without ANY only.
@@ -176,11 +180,18 @@ __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const Ag
         if (tn < full) load_tile<VL>(key, KB, spec, tn * tile + (int64_t)threadIdx.x * kRows, knxt, vnxt);
         if constexpr (kPriv) {
             if (peel_min <= 64) {
+                // lane 0's key takes the registers when at least kPrivLanes
+                // first rows of the step hold it and more than twice as many
+                // as hold the current hot key (so a hot key stays put)
                 const uint64_t k0 = dev::readlane64(kcur[0], 0);
-                if (__popcll(__ballot(kcur[0] == k0)) >= (uint32_t)peel_min && !(priv_on && k0 == hot_key)) {
-                    flush_priv();
-                    hot_key = k0;
-                    priv_on = true;
+                if (!(priv_on && k0 == hot_key)) {
+                    const uint32_t c0 = (uint32_t)__popcll(__ballot(kcur[0] == k0));
+                    const uint32_t ch = priv_on ? (uint32_t)__popcll(__ballot(kcur[0] == hot_key)) : 0u;
+                    if (c0 >= (uint32_t)kPrivLanes && c0 > 2u * ch) {
+                        flush_priv();
+                        hot_key = k0;
+                        priv_on = true;
+                    }
                 }
             }
         }
