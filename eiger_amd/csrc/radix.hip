@@ -5,8 +5,8 @@
 // DeviceScan (operators/sort/sort.cu:15-66, operators/partition/partition.cu:
 // 24-171).  Here one pass over up to 8 digit bits is:
 //   radix_hist_kernel    per 4096-row tile, an LDS histogram of the digit;
-//   scan                 exclusive prefix over [digit][tile] (device-wide,
-//                        the 2^bits digits of the pass only);
+//   offsets              digit-major exclusive prefix of the tile-major
+//                        [tile][digit] counts (coalesced rows, three kernels);
 //   radix_scatter_kernel per tile, a stable rank of every row among the
 //                        tile's rows of the same digit (each wave ranks its own
 //                        1024 consecutive rows with wave64 ballots on the digit
@@ -93,13 +93,13 @@ __global__ __launch_bounds__(kRB) void radix_hist_kernel(RadixKeys k, int64_t n,
         if (e < n) atomicAdd(&h[digit_of(k, raw[i], shift, mask)], 1u);
     }
     __syncthreads();
-    // [digit][tile], only the 2^bits digits in use: each is a lone 4-byte
-    // store into its own digit row
-    if ((uint32_t)t <= mask) hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
+    // [tile][digit], the 2^bits digits in use: one contiguous row per tile
+    if ((uint32_t)t <= mask) hist[(int64_t)blockIdx.x * (mask + 1) + t] = h[t];
 }
 
-// exclusive scan of 256 values held one per thread (256 threads)
-__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wsum)
+// exclusive scan of NT values held one per thread (NT threads; wsum: NT/64 words)
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum)
 {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     uint32_t x = v;
@@ -115,6 +115,8 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wsu
     __syncthreads();
     return x - v + add;
 }
+
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wsum) { return block_excl_scan<256>(v, wsum); }
 
 // A column of the tile leaves in digit order in three steps: its values are
 // loaded into registers (load_column), put at their tile position in LDS
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(kRB) void radix_scatter_kernel(RadixKeys k, void* _
     const int64_t tbase = (int64_t)tile * kRT;
 #pragma unroll
     for (int w = 0; w < kW; ++w) whist[w][t] = 0u;
-    gofs[t] = t < (1 << bits) ? offs[(int64_t)t * ntiles + tile] : 0u;
+    gofs[t] = t < (1 << bits) ? offs[(int64_t)tile * (1 << bits) + t] : 0u;
 
     const int64_t wbase = tbase + (int64_t)wave * kWR;
     uint64_t kr[kRI];
@@ -278,6 +280,69 @@ __global__ __launch_bounds__(kRB) void radix_scatter_kernel(RadixKeys k, void* _
         __syncthreads();
         if (c + 1 < pay.n) load_column(pay.in[c + 1], pay.bytes[c + 1], pay.rowid[c + 1] != 0, pos, wbase, lane, v);
         store_column(pay.out[c], pay.rowid[c] != 0 ? 8 : pay.bytes[c], t, tcount, stage, sdig, gofs, lstart);
+    }
+}
+
+// ---- tile offsets of a radix pass -----------------------------------------------
+// counts[t][d] (tile-major, nd digits) -> offs[t][d] = rows of smaller digits
+// + rows of digit d in tiles before t.  Tiles are taken in chunks of `per`;
+// every access is a contiguous row of nd words (a digit per thread):
+//   1. chunk sums csum[c][d];
+//   2. one workgroup per digit scans its chunk sums (coff[c][d]) and writes
+//      the digit's total;
+//   3. one workgroup per chunk adds the digit's base (a scan of the totals)
+//      and walks its tiles.
+constexpr int kOB = 256;  // threads of the offset kernels (>= digits)
+
+__global__ __launch_bounds__(kOB) void tile_chunk_sums_kernel(const uint32_t* __restrict__ counts, int ntiles, int nd,
+                                                              int per, uint32_t* __restrict__ csum)
+{
+    const int d = threadIdx.x;
+    if (d >= nd) return;
+    const int t0 = blockIdx.x * per, t1 = min(ntiles, t0 + per);
+    uint32_t acc = 0;
+#pragma unroll 8
+    for (int t = t0; t < t1; ++t) acc += counts[(int64_t)t * nd + d];
+    csum[(int64_t)blockIdx.x * nd + d] = acc;
+}
+
+// one workgroup (1024 threads) per digit; nchunks <= 4 * 1024
+__global__ __launch_bounds__(1024) void tile_chunk_scan_kernel(const uint32_t* __restrict__ csum, int nchunks, int nd,
+                                                               uint32_t* __restrict__ coff, uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t wsum[16];
+    const int d = blockIdx.x;
+    uint32_t v[4], run = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int c = threadIdx.x * 4 + i;
+        v[i] = c < nchunks ? csum[(int64_t)c * nd + d] : 0u;
+        run += v[i];
+    }
+    uint32_t ex = block_excl_scan<1024>(run, wsum);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int c = threadIdx.x * 4 + i;
+        if (c < nchunks) coff[(int64_t)c * nd + d] = ex;
+        ex += v[i];
+    }
+    if (threadIdx.x == 1023) total[d] = ex;
+}
+
+__global__ __launch_bounds__(kOB) void tile_offsets_kernel(const uint32_t* __restrict__ counts, int ntiles, int nd,
+                                                           int per, const uint32_t* __restrict__ coff,
+                                                           const uint32_t* __restrict__ total, uint32_t* __restrict__ offs)
+{
+    __shared__ uint32_t wsum[kOB / 64];
+    const int d = threadIdx.x;
+    const uint32_t base = block_excl_scan<kOB>(d < nd ? total[d] : 0u, wsum);
+    if (d >= nd) return;
+    const int t0 = blockIdx.x * per, t1 = min(ntiles, t0 + per);
+    uint32_t run = base + coff[(int64_t)blockIdx.x * nd + d];
+    for (int t = t0; t < t1; ++t) {
+        const uint32_t c = counts[(int64_t)t * nd + d];
+        offs[(int64_t)t * nd + d] = run;
+        run += c;
     }
 }
 
@@ -422,7 +487,8 @@ void radix_pass(const RadixKeys& k, void* keys_out, const std::vector<Payload>& 
     if ((int)pay.size() > kMaxPay) raise(EG_ERR_UNSUPPORTED, "radix pass: too many payload columns");
     const int ntiles = (int)((n + kRT - 1) / kRT);
     const uint32_t mask = (1u << bits) - 1u;
-    const int64_t nh = ((int64_t)1 << bits) * ntiles;  // [digit][tile] counts
+    const int nd = 1 << bits;
+    const int64_t nh = (int64_t)nd * ntiles;  // [tile][digit] counts
     DevBuf hist((size_t)nh * 4, s), offs((size_t)nh * 4, s);
     {
         KernelTimer timer("radix_hist", s);
@@ -433,7 +499,21 @@ void radix_pass(const RadixKeys& k, void* keys_out, const std::vector<Payload>& 
     }
     {
         KernelTimer timer("radix_scan", s);
-        scan_exclusive_u32(hist.get<uint32_t>(), offs.get<uint32_t>(), nh, s, false);
+        const int per = std::max(16, (ntiles + 4095) / 4096);  // tiles per chunk: at most 4096 chunks
+        const int nchunks = (ntiles + per - 1) / per;
+        DevBuf cs((size_t)nchunks * nd * 4 * 2 + (size_t)nd * 4, s);
+        uint32_t* csum = cs.get<uint32_t>();
+        uint32_t* coff = csum + (size_t)nchunks * nd;
+        uint32_t* total = coff + (size_t)nchunks * nd;
+        hipLaunchKernelGGL(tile_chunk_sums_kernel, dim3(nchunks), dim3(kOB), 0, s, (const uint32_t*)hist.get<uint32_t>(),
+                           ntiles, nd, per, csum);
+        EG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(tile_chunk_scan_kernel, dim3(nd), dim3(1024), 0, s, (const uint32_t*)csum, nchunks, nd, coff,
+                           total);
+        EG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(tile_offsets_kernel, dim3(nchunks), dim3(kOB), 0, s, (const uint32_t*)hist.get<uint32_t>(),
+                           ntiles, nd, per, (const uint32_t*)coff, (const uint32_t*)total, offs.get<uint32_t>());
+        EG_LAUNCH_CHECK();
     }
     PayArgs pa{};
     pa.n = (int32_t)pay.size();
