@@ -648,14 +648,28 @@ __device__ __forceinline__ void part_agg(const KeyDesc& key, const AggSpec& spec
         return sl == tab.cr && tab.side[0] != 0ull;
     };
     if (u.sliced) {
-        // the partition continues in other units: merge into the HBM table
-        for (int sl = threadIdx.x; sl <= tab.cr; sl += blockDim.x) {
+        // the partition continues in other units: merge into the HBM table.
+        // Each lane finds its slot's HBM slot, then groups of 8 lanes (16
+        // above 8 ops) merge one slot each, lane j its state j: one request
+        // per op class and slot (see hbm_grouped).
+        const int nops = OPS::n(spec);
+        const int lg = nops <= 8 ? 3 : 4;
+        const int grp = lane >> lg, j = lane & ((1 << lg) - 1);
+        for (int base = 0; base <= tab.cr; base += blockDim.x) {
+            const int sl = base + threadIdx.x;
             KT kw;
-            if (!slot_key(sl, kw)) continue;
-            unsigned long long* g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed), ins);
-            if (g)
-                dev::merge_states<OPS>(spec, [&](int a) { return g + 1 + a; },
-                                       [&](int a) { return tab.st[(size_t)a * tab.T + sl]; });
+            unsigned long long* g = nullptr;
+            if (sl <= tab.cr && slot_key(sl, kw)) g = gt_slot(gt, lds_to_enc64<KB>(kw, key.is_signed), ins);
+            const uint64_t m = __ballot(g != nullptr);
+            const int np = __popcll(m);
+            for (int r = 0; r < np; r += 64 >> lg) {
+                const int q = r + grp;
+                const int src = q < np ? nth_set_bit(m, q) : 0;
+                unsigned long long* d = (unsigned long long*)__shfl((unsigned long long)(uintptr_t)g, src);
+                const int ssl = __shfl(sl, src);
+                if (q < np && j < nops)
+                    dev::apply_atomic(OPS::cls(spec, j), d + 1 + j, (uint64_t)tab.st[(size_t)j * tab.T + ssl]);
+            }
         }
     } else {
         // Written straight out: one claim per unit on its lane (see
