@@ -131,6 +131,37 @@ __global__ __launch_bounds__(256) void part_compact_kernel(GTable gt, OutSpec os
     });
 }
 
+// Units from the partition boundaries, on the device: a partition of at most
+// `slice` rows is one unit, a larger one ceil(rows / slice) sliced units.
+// units_count_kernel counts each partition's units (cnt) and the sliced
+// partitions (meta[0]); after an exclusive scan of cnt (pre),
+// units_fill_kernel writes them and the unit total (meta[1]).
+__global__ __launch_bounds__(256) void units_count_kernel(const int64_t* __restrict__ offs, int64_t P, int64_t slice,
+                                                          uint32_t* __restrict__ cnt, unsigned long long* meta)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int64_t rows = offs[p + 1] - offs[p];
+    cnt[p] = rows <= 0 ? 0u : rows <= slice ? 1u : (uint32_t)((rows + slice - 1) / slice);
+    if (rows > slice) atomicAdd(&meta[0], 1ull);
+}
+
+__global__ __launch_bounds__(256) void units_fill_kernel(const int64_t* __restrict__ offs, int64_t P, int64_t slice,
+                                                         const uint32_t* __restrict__ cnt,
+                                                         const uint32_t* __restrict__ pre, agg::Unit* __restrict__ units,
+                                                         unsigned long long* meta)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const uint32_t k = cnt[p];
+    const int64_t a = offs[p], e = offs[p + 1];
+    for (uint32_t i = 0; i < k; ++i) {
+        const int64_t x = a + (int64_t)i * slice;
+        units[pre[p] + i] = agg::Unit{x, k == 1 ? e : (x + slice < e ? x + slice : e), k > 1 ? 1 : 0, 0};
+    }
+    if (p == P - 1) meta[1] = (unsigned long long)pre[p] + k;
+}
+
 // Output rows past the group count moved into the holes below it (the
 // unused tails of the lanes' last chunks): op i copies len[i] rows from
 // src[i] to dst[i] in every output column; sources and destinations are
@@ -313,9 +344,6 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
         else hipLaunchKernelGGL(part_bounds_kernel<8>, dim3((unsigned)grid), dim3(256), 0, s, pkeys, n, pmask, offs.get<int64_t>());
         EG_LAUNCH_CHECK();
     }
-    int64_t* ho = (int64_t*)pinned_staging((size_t)(P + 1) * 8, 0);
-    EG_HIP(hipMemcpyAsync(ho, offs.get(), (size_t)(P + 1) * 8, hipMemcpyDeviceToHost, s));
-    EG_HIP(hipStreamSynchronize(s));
     // Rows per unit: a partition is split when it is much larger than the
     // average (skew), or when there are too few partitions to occupy every CU,
     // since a split partition's groups merge through the HBM table with
@@ -323,31 +351,37 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
     const int64_t avg_rows = (n + P - 1) / P;
     const int64_t balance = std::max<int64_t>(1, n / (4 * (int64_t)di.num_cus));
     const int64_t slice = std::max<int64_t>({(int64_t)tune.block * 16, 8 * per_part, std::min(4 * avg_rows, balance)});
-    std::vector<Unit> units;
-    for (int64_t p = 0; p < P; ++p) {
-        const int64_t a = ho[p], e = ho[p + 1];
-        if (e <= a) continue;
-        if (e - a <= slice) {
-            units.push_back(Unit{a, e, 0, 0});
-        } else {
-            for (int64_t x = a; x < e; x += slice) units.push_back(Unit{x, std::min(e, x + slice), 1, 0});
-        }
-    }
-    if (units.size() >= ((size_t)1 << 31)) raise(EG_ERR_UNSUPPORTED, "too many partition units");
-    DevBuf dunits(units.size() * sizeof(Unit), s);
+    // units built on the device; 16 bytes (unit and sliced-partition counts)
+    // come back instead of the boundaries, and no unit list goes up
+    const int64_t max_units = P + n / slice + 1;
+    if (max_units >= ((int64_t)1 << 31)) raise(EG_ERR_UNSUPPORTED, "too many partition units");
+    DevBuf dunits((size_t)max_units * sizeof(Unit), s);
+    int64_t nunits = 0, sliced_parts = 0;
     {
-        // staged through pinned memory: the copy is a DMA, not a pageable bounce
-        void* hu = pinned_staging(units.size() * sizeof(Unit), 1);
-        std::memcpy(hu, units.data(), units.size() * sizeof(Unit));
-        EG_HIP(hipMemcpyAsync(dunits.get(), hu, units.size() * sizeof(Unit), hipMemcpyHostToDevice, s));
+        DevBuf ub((size_t)P * 8 + 16, s);
+        uint32_t* cnt = ub.get<uint32_t>();
+        uint32_t* pre = cnt + P;
+        unsigned long long* umeta = reinterpret_cast<unsigned long long*>(pre + P);
+        EG_HIP(hipMemsetAsync(umeta, 0, 16, s));
+        const unsigned ugrid = (unsigned)((P + 255) / 256);
+        hipLaunchKernelGGL(units_count_kernel, dim3(ugrid), dim3(256), 0, s, (const int64_t*)offs.get<int64_t>(), P,
+                           slice, cnt, umeta);
+        EG_LAUNCH_CHECK();
+        scan_exclusive_u32(cnt, pre, P, s, false);
+        hipLaunchKernelGGL(units_fill_kernel, dim3(ugrid), dim3(256), 0, s, (const int64_t*)offs.get<int64_t>(), P,
+                           slice, (const uint32_t*)cnt, (const uint32_t*)pre, dunits.get<Unit>(), umeta);
+        EG_LAUNCH_CHECK();
+        int64_t* hm = pinned_i64(2);
+        EG_HIP(hipMemcpyAsync(hm, umeta, 16, hipMemcpyDeviceToHost, s));
+        EG_HIP(hipStreamSynchronize(s));
+        sliced_parts = hm[0];
+        nunits = hm[1];
     }
 
     phase_units.reset();
     HostPhase phase_agg("host_partition_aggregate", s);
     // ---- aggregation ------------------------------------------------------------------
     // HBM table: groups of sliced partitions plus LDS spills
-    int64_t sliced_parts = 0;
-    for (int64_t p = 0; p < P; ++p) sliced_parts += (ho[p + 1] - ho[p]) > slice ? 1 : 0;
     int64_t capacity = bound;
     int64_t slots = 1 << 14;
     // sliced partitions' groups, plus keys the LDS tables may not hold (a
@@ -376,10 +410,10 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
         fill_zero(cbuf.get(), claim_bytes, s);
         claims.words = cbuf.get<unsigned long long>();
         if (kb == 4)
-            launch_part<4>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os, claims,
+            launch_part<4>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)nunits, ht.gt, os, claims,
                            cap_rows, cr, lds, tune.block, n, s);
         else
-            launch_part<8>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)units.size(), ht.gt, os, claims,
+            launch_part<8>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)nunits, ht.gt, os, claims,
                            cap_rows, cr, lds, tune.block, n, s);
         int64_t meta[3];
         ht.read_meta(meta, s);
