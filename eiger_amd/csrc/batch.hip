@@ -286,8 +286,8 @@ __global__ __launch_bounds__(kBatchBlock, 8) void batch_agg_kernel(const BQuery*
     __shared__ unsigned long long sh[17];
     const int nops = spec.nops;
     const int lane = dev::lane_id();
-    LdsTable<KB, true> tab(smem, nops, cr_log2, 0);
-    const int words = LdsTable<KB>::words(nops, cr_log2, 0);
+    LdsTable<KB, true> tab(smem, nops, 1 << cr_log2, 0);
+    const int words = LdsTable<KB>::words(nops, 1 << cr_log2, 0);
     uint32_t ins = 0;
     dev::HotTags hot;  // groups the wave combined last
     KeyDesc key;
@@ -535,7 +535,7 @@ BatchPlan* batch_plan_create(const eg_batch_query* queries, int64_t nq, int32_t 
     // LDS table of the aggregation kernel (slots clamped to what fits)
     const DeviceInfo& di = device_info();
     int cr = T.lds_slots_log2;
-    auto lds_of = [&](int c) { return (size_t)lds::lds_table_words(p->spec.nops, c, 0, p->kb) * 8; };
+    auto lds_of = [&](int c) { return (size_t)lds::lds_table_words(p->spec.nops, 1 << c, 0, p->kb) * 8; };
     while (cr > 6 && lds_of(cr) + 256 > (size_t)di.max_lds) --cr;
     p->tuning.lds_slots_log2 = cr;
     p->lds = lds_of(cr);

@@ -108,20 +108,20 @@ constexpr int kPrivLanes = 8;
 
 template <int KB, class VL, class OPS>
 __device__ __forceinline__ void hash_agg(const KeyDesc& key, int64_t n, const AggSpec& spec, const GTable& gt,
-                                         int cr_log2, int rep_log2, int probe_limit, int peel_min)
+                                         int cr, int rep_log2, int probe_limit, int peel_min)
 {
     constexpr int NV = VL::NV > 0 ? VL::NV : 1;
     extern __shared__ __align__(16) unsigned long long smem[];
-    const bool use_lds = cr_log2 >= 0;
+    const bool use_lds = cr > 0;
     // Rows reach the HBM table as spills of cold keys (hot keys take LDS
     // slots first), where combining seldom finds partners; only without an
     // LDS stage do hot keys meet there.
     const int hbm_peel = use_lds ? 65 : peel_min;
     const int nops = OPS::n(spec);
     const int lane = dev::lane_id();
-    LdsTable<KB> tab(smem, nops, use_lds ? cr_log2 : 0, use_lds ? rep_log2 : 0);
+    LdsTable<KB> tab(smem, nops, use_lds ? cr : 1, use_lds ? rep_log2 : 0);
     if (use_lds) {
-        const int words = LdsTable<KB>::words(nops, cr_log2, rep_log2);
+        const int words = LdsTable<KB>::words(nops, cr, rep_log2);
         for (int i = threadIdx.x; i < words; i += blockDim.x) smem[i] = 0ull;
         __syncthreads();
     }

@@ -62,21 +62,23 @@ __device__ __forceinline__ unsigned long long* gt_slot(const GTable& gt, uint64_
 
 // Every group of the table (slots, then the side slot) handed to
 // emit(index, enc, states), indices dense from *counter on.  A thread looks
-// at kCompactSlots slots per round and a block claims its round's groups with
-// one device atomic: same-address atomics serialise (~80 per microsecond), so
-// one claim per 256 slots cost a 3M-slot table ~0.2 ms.
+// at R slots per round and a block claims its round's groups with one device
+// atomic: same-address atomics serialise (~80 per microsecond), so one claim
+// per 256 slots cost a 3M-slot table ~0.2 ms.  Large tables take
+// kCompactSlots per thread; small ones one (more workgroups, shorter chains).
 constexpr int kCompactSlots = 8;
-template <class EMIT>
+constexpr uint64_t kCompactWideSlots = 1u << 16;  // tables from this size take kCompactSlots
+template <int R, class EMIT>
 __device__ __forceinline__ void compact_groups(const GTable& gt, unsigned long long* counter, unsigned long long* sh,
                                                EMIT emit)
 {
     const uint64_t total = gt.mask + 2;  // slots + side slot
-    const uint64_t chunk = (uint64_t)blockDim.x * kCompactSlots;
+    const uint64_t chunk = (uint64_t)blockDim.x * R;
     for (uint64_t base = (uint64_t)blockIdx.x * chunk; base < total; base += (uint64_t)gridDim.x * chunk) {
-        uint64_t enc[kCompactSlots];
+        uint64_t enc[R];
         uint32_t occ = 0;
 #pragma unroll
-        for (int i = 0; i < kCompactSlots; ++i) {
+        for (int i = 0; i < R; ++i) {
             const uint64_t s = base + (uint64_t)i * blockDim.x + threadIdx.x;
             enc[i] = 0;
             if (s <= gt.mask) {
@@ -88,7 +90,7 @@ __device__ __forceinline__ void compact_groups(const GTable& gt, unsigned long l
         }
         int64_t idx = dev::block_claim_n(counter, (uint32_t)__popc(occ), sh);
 #pragma unroll
-        for (int i = 0; i < kCompactSlots; ++i) {
+        for (int i = 0; i < R; ++i) {
             if (!((occ >> i) & 1u)) continue;
             const uint64_t s = base + (uint64_t)i * blockDim.x + threadIdx.x;
             const unsigned long long* slot = s <= gt.mask ? gt.base + s * (uint64_t)gt.W : gt.side;
@@ -121,7 +123,7 @@ struct HbmTable {
 
 // LDS table geometry of the hash kernel.
 struct HashLayout {
-    int cr_log2 = -1;     // slots per replica (log2), -1 = no LDS stage
+    int cr = -1;          // slots per replica, -1 = no LDS stage
     int rep_log2 = 0;     // replicas (log2)
     size_t lds_bytes = 0;
     int probe_limit = 16;

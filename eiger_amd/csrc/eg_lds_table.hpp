@@ -159,10 +159,9 @@ __device__ __forceinline__ void load_tile(const KeyDesc& key, int kb, const AggS
 // aligned.
 __host__ __device__ constexpr int lds_keys_word(int nops, int T) { return (nops * T + 1) & ~1; }
 
-__host__ __device__ constexpr int lds_table_words(int nops, int cr_log2, int rep_log2, int kb)
+__host__ __device__ constexpr int lds_table_words(int nops, int cr, int rep_log2, int kb)
 {
-    return lds_keys_word(nops, ((1 << cr_log2) + 1) << rep_log2) +
-           ((((1 << cr_log2) + 1) << rep_log2) * kb + 7) / 8 + (1 << rep_log2);
+    return lds_keys_word(nops, (cr + 1) << rep_log2) + (((cr + 1) << rep_log2) * kb + 7) / 8 + (1 << rep_log2);
 }
 
 // Probing of the unreplicated table (R = 1) reads keys kBucket at a time
@@ -186,29 +185,31 @@ struct LdsTable {
     unsigned long long* st;
     KT* keys;
     unsigned long long* side;
-    int T, cr, rlog, shift;
+    int T, cr, rlog;
 
-    __device__ LdsTable(unsigned long long* smem, int nops, int cr_log2, int rep_log2)
+    // cr slots per replica: a power of two when replicated, else any
+    // multiple of kBucket (a 1-slot table is never probed)
+    __device__ LdsTable(unsigned long long* smem, int nops, int cr_slots, int rep_log2)
     {
-        cr = 1 << cr_log2;
-        shift = cr_log2 > 0 ? 32 - cr_log2 : 31;  // a 1-slot table is never probed
+        cr = cr_slots;
         rlog = rep_log2;
         T = (cr + 1) << rlog;
         st = smem;
         keys = reinterpret_cast<KT*>(st + lds_keys_word(nops, T));
         side = st + lds_keys_word(nops, T) + ((size_t)T * KB + 7) / 8;
     }
-    __device__ static int words(int nops, int cr_log2, int rep_log2)
+    __device__ static int words(int nops, int cr, int rep_log2)
     {
-        return lds_table_words(nops, cr_log2, rep_log2, KB);
+        return lds_table_words(nops, cr, rep_log2, KB);
     }
-    // Home slot of encoded key e: multiplicative (Fibonacci) hashing, the top
-    // cr_log2 bits of e * 2^32/phi -- one multiply; 8-byte keys fold their
-    // halves first.  Only probe lengths depend on it (1 <= cr_log2 <= 31).
+    // Home slot of encoded key e: multiplicative (Fibonacci) hashing scaled to
+    // the table, the high word of (e * 2^32/phi) * cr -- for cr = 2^k the top
+    // k bits of the product; 8-byte keys fold their halves first.  Only probe
+    // lengths depend on it.
     __device__ __forceinline__ uint32_t home(KT e) const
     {
         const uint32_t x = (KB == 4) ? (uint32_t)e : (uint32_t)e ^ (uint32_t)((uint64_t)e >> 32);
-        return (x * 0x9e3779b1u) >> shift;
+        return __umulhi(x * 0x9e3779b1u, (uint32_t)cr);
     }
     // The kBucket keys from slot b on (R = 1, b a multiple of kBucket).
     __device__ __forceinline__ void read_bucket(uint32_t b, KT (&k)[kBucket]) const
@@ -254,7 +255,7 @@ struct LdsTable {
                     if (prev == 0 || prev == e) return (int)b + i;
                 }
                 left -= kBucket - i0;
-                h = (b + kBucket) & (uint32_t)(cr - 1);
+                h = b + kBucket >= (uint32_t)cr ? 0u : b + kBucket;
             }
             return -1;
         }
@@ -266,7 +267,7 @@ struct LdsTable {
                 KT prev = atomicCAS(&keys[idx], (KT)0, e);
                 if (prev == 0 || prev == e) return idx;
             }
-            h = (h + 1) & (uint32_t)(cr - 1);
+            h = h + 1 == (uint32_t)cr ? 0u : h + 1;
         }
         return -1;
     }

@@ -46,18 +46,33 @@ using agg::max_block;
 
 template <int KB, class VL>
 __global__ __launch_bounds__(max_block<VL::NV>()) void hash_agg_kernel(KeyDesc key, int64_t n, AggSpec spec,
-                                                                        GTable gt, int cr_log2, int rep_log2,
+                                                                        GTable gt, int cr, int rep_log2,
                                                                         int probe_limit, int peel_min)
 {
-    agg::hash_agg<KB, VL, dev::RtOps>(key, n, spec, gt, cr_log2, rep_log2, probe_limit, peel_min);
+    agg::hash_agg<KB, VL, dev::RtOps>(key, n, spec, gt, cr, rep_log2, probe_limit, peel_min);
 }
 
+template <int R>
 __global__ __launch_bounds__(256) void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter)
 {
     __shared__ unsigned long long sh[17];
-    compact_groups(gt, counter, sh, [&](int64_t idx, uint64_t enc, const unsigned long long* st) {
+    compact_groups<R>(gt, counter, sh, [&](int64_t idx, uint64_t enc, const unsigned long long* st) {
         dev::emit_group(os, idx, enc, st, 1);
     });
+}
+
+// Compaction of gt into os from *counter on (grid and slots per thread by size).
+void launch_compact(const GTable& gt, const OutSpec& os, unsigned long long* counter, hipStream_t s)
+{
+    const DeviceInfo& di = device_info();
+    const uint64_t total = gt.mask + 2;
+    const bool wide = total >= kCompactWideSlots;
+    const int64_t per = 256 * (wide ? kCompactSlots : 1);
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + per - 1) / per), (int64_t)di.num_cus * 8));
+    KernelTimer timer("hash_compact", s);
+    if (wide) hipLaunchKernelGGL(hash_compact_kernel<kCompactSlots>, dim3((unsigned)grid), dim3(256), 0, s, gt, os, counter);
+    else hipLaunchKernelGGL(hash_compact_kernel<1>, dim3((unsigned)grid), dim3(256), 0, s, gt, os, counter);
+    EG_LAUNCH_CHECK();
 }
 
 // use_jit: take the specialised kernel once it is loaded (loading a module
@@ -83,14 +98,14 @@ void launch_agg(const DeviceInfo& di, const HashTuning& t, const KeyDesc& key, i
         int64_t a_n = n;
         AggSpec a_spec = spec;
         GTable a_gt = gt;
-        int a_cr = L.cr_log2, a_rep = L.rep_log2, a_probe = L.probe_limit, a_peel = L.peel_min;
+        int a_cr = L.cr, a_rep = L.rep_log2, a_probe = L.probe_limit, a_peel = L.peel_min;
         void* args[] = {&a_key, &a_n, &a_spec, &a_gt, &a_cr, &a_rep, &a_probe, &a_peel};
         if (jit::launch(f, (unsigned)grid, (unsigned)block, L.lds_bytes, s, args)) return;
     }
     int per_cu = t.blocks_per_cu;
     if (per_cu <= 0) EG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, L.lds_bytes));
     const int64_t grid = grid_for(per_cu);
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), L.lds_bytes, s, key, n, spec, gt, L.cr_log2,
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), L.lds_bytes, s, key, n, spec, gt, L.cr,
                        L.rep_log2, L.probe_limit, L.peel_min);
     EG_LAUNCH_CHECK();
 }
@@ -113,7 +128,7 @@ HashLayout plan_hash_layout(int nops, int key_bytes, int64_t group_bound, const 
     const DeviceInfo& di = device_info();
     HashLayout L;
     auto bytes_of = [&](int cr_log2, int rep_log2) {
-        return (size_t)lds::lds_table_words(nops, cr_log2, rep_log2, key_bytes) * 8;
+        return (size_t)lds::lds_table_words(nops, 1 << cr_log2, rep_log2, key_bytes) * 8;
     };
     const size_t budget = (size_t)di.max_lds;
     // replicas: the most (<= 64) that still give every replica >= 2x the group bound
@@ -134,13 +149,20 @@ HashLayout plan_hash_layout(int nops, int key_bytes, int64_t group_bound, const 
         }
     }
     if (best_c >= 0 && group_bound <= (int64_t)t.lds_max_ratio * ((int64_t)1 << best_c)) {
-        L.cr_log2 = best_c;
+        L.cr = 1 << best_c;
         L.rep_log2 = best_r;
-        L.lds_bytes = bytes_of(best_c, best_r);
+        if (best_r == 0) {
+            // one table: every multiple of four slots the LDS holds (a larger
+            // table keeps more of the keys that arrive, and fewer rows spill)
+            auto bytes_cr = [&](int cr) { return (size_t)lds::lds_table_words(nops, cr, 0, key_bytes) * 8; };
+            const int cap = t.lds_slots > 0 ? t.lds_slots : (1 << 14);
+            while (L.cr + 4 <= cap && bytes_cr(L.cr + 4) + 1024 <= budget) L.cr += 4;  // 1 KB for static LDS
+        }
+        L.lds_bytes = (size_t)lds::lds_table_words(nops, L.cr, L.rep_log2, key_bytes) * 8;
     }
     // A table much smaller than the group count still catches the hot keys
     // (they arrive early); keep the probing short so cold rows spill cheaply.
-    const int64_t slots = (int64_t)1 << std::max(best_c, 0);
+    const int64_t slots = std::max(L.cr, 1);
     L.probe_limit = group_bound > 4 * slots ? 2 : group_bound > slots ? 4 : 16;
     L.peel_min = t.peel_min;
     return L;
@@ -274,13 +296,7 @@ int64_t run_hash_noalloc(const Plan& plan, int64_t group_bound, const OutSpec& o
             raise(EG_ERR_CAPACITY, std::to_string(groups) + " groups exceed the output capacity of " +
                                        std::to_string(capacity) + " rows");
         if (groups > 0) {
-            const uint64_t total = gt.mask + 2;
-            const int64_t per = 256 * kCompactSlots;
-            const int64_t grid =
-                std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + per - 1) / per), (int64_t)di.num_cus * 8));
-            KernelTimer timer("hash_compact", s);
-            hipLaunchKernelGGL(hash_compact_kernel, dim3((unsigned)grid), dim3(256), 0, s, gt, os, gt.meta + 3);
-            EG_LAUNCH_CHECK();
+            launch_compact(gt, os, gt.meta + 3, s);
         }
         EG_HIP(hipStreamSynchronize(s));
         return groups;
@@ -319,13 +335,7 @@ void HbmTable::read_meta(int64_t* out3, hipStream_t s)
 
 void HbmTable::compact(const OutSpec& os, unsigned long long* counter, hipStream_t s)
 {
-    const DeviceInfo& di = device_info();
-    const uint64_t total = gt.mask + 2;
-    const int64_t per = 256 * kCompactSlots;
-    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + per - 1) / per), (int64_t)di.num_cus * 8));
-    KernelTimer timer("hash_compact", s);
-    hipLaunchKernelGGL(hash_compact_kernel, dim3((unsigned)grid), dim3(256), 0, s, gt, os, counter);
-    EG_LAUNCH_CHECK();
+    launch_compact(gt, os, counter, s);
 }
 
 }  // namespace eg

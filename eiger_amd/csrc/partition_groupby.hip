@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void part_compact_kernel(GTable gt, OutSpec os
                                                            int64_t capacity)
 {
     __shared__ unsigned long long sh[17];
-    compact_groups(gt, counter, sh, [&](int64_t idx, uint64_t enc, const unsigned long long* st) {
+    compact_groups<kCompactSlots>(gt, counter, sh, [&](int64_t idx, uint64_t enc, const unsigned long long* st) {
         if (idx < capacity) dev::emit_group(os, idx, enc, st, 1);
     });
 }
