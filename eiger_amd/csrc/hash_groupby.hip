@@ -52,17 +52,21 @@ __global__ __launch_bounds__(max_block<VL::NV>()) void hash_agg_kernel(KeyDesc k
     agg::hash_agg<KB, VL, dev::RtOps>(key, n, spec, gt, cr, rep_log2, probe_limit, peel_min);
 }
 
+// Groups at indices from `capacity` on are not written (the caller sees the
+// count and re-runs the compaction with room for them).
 template <int R>
-__global__ __launch_bounds__(256) void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter)
+__global__ __launch_bounds__(256) void hash_compact_kernel(GTable gt, OutSpec os, unsigned long long* counter,
+                                                           int64_t capacity)
 {
     __shared__ unsigned long long sh[17];
     compact_groups<R>(gt, counter, sh, [&](int64_t idx, uint64_t enc, const unsigned long long* st) {
-        dev::emit_group(os, idx, enc, st, 1);
+        if (idx < capacity) dev::emit_group(os, idx, enc, st, 1);
     });
 }
 
 // Compaction of gt into os from *counter on (grid and slots per thread by size).
-void launch_compact(const GTable& gt, const OutSpec& os, unsigned long long* counter, hipStream_t s)
+void launch_compact(const GTable& gt, const OutSpec& os, unsigned long long* counter, hipStream_t s,
+                    int64_t capacity = INT64_MAX)
 {
     const DeviceInfo& di = device_info();
     const uint64_t total = gt.mask + 2;
@@ -70,8 +74,10 @@ void launch_compact(const GTable& gt, const OutSpec& os, unsigned long long* cou
     const int64_t per = 256 * (wide ? kCompactSlots : 1);
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + per - 1) / per), (int64_t)di.num_cus * 8));
     KernelTimer timer("hash_compact", s);
-    if (wide) hipLaunchKernelGGL(hash_compact_kernel<kCompactSlots>, dim3((unsigned)grid), dim3(256), 0, s, gt, os, counter);
-    else hipLaunchKernelGGL(hash_compact_kernel<1>, dim3((unsigned)grid), dim3(256), 0, s, gt, os, counter);
+    if (wide)
+        hipLaunchKernelGGL(hash_compact_kernel<kCompactSlots>, dim3((unsigned)grid), dim3(256), 0, s, gt, os, counter,
+                           capacity);
+    else hipLaunchKernelGGL(hash_compact_kernel<1>, dim3((unsigned)grid), dim3(256), 0, s, gt, os, counter, capacity);
     EG_LAUNCH_CHECK();
 }
 
@@ -168,6 +174,10 @@ HashLayout plan_hash_layout(int nops, int key_bytes, int64_t group_bound, const 
     return L;
 }
 
+// Group bounds up to which run_hash allocates the outputs before it reads
+// the group count (at most 40 bytes a group for the benchmark's requests).
+constexpr int64_t kEarlyOutputRows = (int64_t)1 << 22;
+
 Result run_hash(const Plan& plan, int64_t group_bound, hipStream_t s, const HashTuning& t)
 {
     const DeviceInfo& di = device_info();
@@ -184,10 +194,21 @@ Result run_hash(const Plan& plan, int64_t group_bound, hipStream_t s, const Hash
     int64_t slots = t.table_slots > 0 ? t.table_slots : 1024;
     while (slots < 2 * bound) slots <<= 1;
 
+    // With a modest bound the outputs are allocated for it before the table is
+    // read, and the compaction runs right after the aggregation: one
+    // synchronisation per call instead of two (a bound the groups exceed
+    // costs one more compaction into exact outputs).
+    const bool early = bound <= kEarlyOutputRows;
     for (;;) {
         HbmTable ht(slots, W, s);
         if (KB == 4) dispatch_layout<4>(vec, di, t, plan.key, n, plan.spec, ht.gt, L, s);
         else dispatch_layout<8>(vec, di, t, plan.key, n, plan.spec, ht.gt, L, s);
+        Result eres;
+        OutSpec eos;
+        if (early) {
+            alloc_outputs(plan, bound, s, eres, eos);
+            launch_compact(ht.gt, eos, ht.gt.meta + 3, s, bound);  // meta[3]: zeroed with the table
+        }
         int64_t meta[3];
         ht.read_meta(meta, s);
         if (meta[2] != 0) {
@@ -201,6 +222,10 @@ Result run_hash(const Plan& plan, int64_t group_bound, hipStream_t s, const Hash
             continue;
         }
         const int64_t groups = meta[0] + (meta[1] != 0 ? 1 : 0);
+        if (early && groups <= bound) {
+            eres.num_groups = groups;
+            return eres;  // read_meta synchronised after the compaction
+        }
         Result res;
         OutSpec os;
         alloc_outputs(plan, groups, s, res, os);
