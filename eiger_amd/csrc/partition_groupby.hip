@@ -415,11 +415,10 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
         else
             launch_part<8>(gfur, pkey, spec, perm, dunits.get<Unit>(), (int)nunits, ht.gt, os, claims,
                            cap_rows, cr, lds, tune.block, n, s);
-        int64_t meta[3];
-        ht.read_meta(meta, s);
-        const bool table_full = meta[2] != 0;
-        if (!table_full) {
-            // append HBM-table groups after every claimed chunk
+        {
+            // append HBM-table groups after every claimed chunk (also when
+            // the table filled up: the outputs are then dropped for the
+            // retry below, and the capacity bounds every write)
             const uint64_t total = ht.gt.mask + 2;
             const int64_t per = 256 * kCompactSlots;
             const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)((total + per - 1) / per), (int64_t)di.num_cus * 8));
@@ -428,9 +427,14 @@ Result run_partition(const Plan& plan, int64_t group_bound, bool gfur, hipStream
                                claims.words, cap_rows);
             EG_LAUNCH_CHECK();
         }
+        // the claims' head and the table's meta words in one synchronisation
         int64_t* h = (int64_t*)pinned_staging((size_t)agg::kClaimHeadWords * 8, 2);
+        int64_t* hm = pinned_i64(4);
         EG_HIP(hipMemcpyAsync(h, claims.words, (size_t)agg::kClaimHeadWords * 8, hipMemcpyDeviceToHost, s));
+        EG_HIP(hipMemcpyAsync(hm, ht.gt.meta, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         EG_HIP(hipStreamSynchronize(s));
+        const int64_t meta[3] = {hm[0], hm[1], hm[2]};
+        const bool table_full = meta[2] != 0;
         const int64_t total = h[0];  // output rows claimed: chunks, then the HBM table's groups
         if (table_full || total > cap_rows) {
             // the HBM table or the output ran out: retry with room for every row
