@@ -223,6 +223,20 @@ def test_group_bound_far_too_small_retries(name):
     assert_same(got, O.groupby(keys, [vals], reqs))
 
 
+@pytest.mark.parametrize("name", ["hash", "partition"])
+def test_group_bound_below_count_without_table_overflow(name):
+    """A bound somewhat below the true group count: the table (sized from the
+    bound) still holds every group, but outputs sized by the bound would not;
+    every group comes back."""
+    rng = np.random.default_rng(23)
+    n = 1 << 17
+    keys = rng.integers(0, 1500, n).astype(np.int32)
+    vals = rng.integers(-1000, 1000, n).astype(np.int64)
+    reqs = [(-1, A.COUNT_ALL), (0, A.SUM), (0, A.MIN)]
+    got = sorted_by_key(run(STRATEGIES[name], keys, [vals], reqs, G.groupby_hint(groups_upper_bound=1000)))
+    assert_same(got, O.groupby(keys, [vals], reqs))
+
+
 @pytest.mark.parametrize("name", ["sort", "sort_gfur"])
 def test_sort_strategy_output_is_key_ordered(name):
     rng = np.random.default_rng(8)
