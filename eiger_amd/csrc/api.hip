@@ -331,8 +331,7 @@ int groupby_entry(Entry entry, const eg_column_view* keys, int64_t num_keys,
             std::vector<ViewIn> kv;
             for (int64_t i = 0; i < num_keys; ++i) kv.push_back(view_in(keys[i]));
             FusedKeys f;
-            if (!fuse_keys(kv.data(), num_keys, s, f))
-                raise(EG_ERR_UNSUPPORTED, "multi-column keys need more than 64 bits after range reduction");
+            fuse_keys(kv.data(), num_keys, s, f);
             eg_column_view fk{f.fused.get(), f.rows, EG_UINT64, 0, 0};
             eg_groupby_hint h2 = h;
             if (f.total_bits < 62 && h2.groups_upper_bound > ((int64_t)1 << f.total_bits))

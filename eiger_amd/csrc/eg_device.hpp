@@ -479,5 +479,25 @@ __device__ __forceinline__ int64_t block_claim_n(unsigned long long* counter, ui
     return r;
 }
 
+// Exclusive scan of NT values held one per thread (NT threads; wsum: NT/64
+// words of LDS).
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int i = 0; i < w; ++i) add += wsum[i];
+    __syncthreads();
+    return x - v + add;
+}
+
 }  // namespace dev
 }  // namespace eg

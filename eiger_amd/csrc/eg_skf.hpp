@@ -11,22 +11,43 @@
 namespace eg {
 
 constexpr int kMaxFusedCols = 8;
+constexpr int kMaxFuseItems = kMaxFusedCols + 1;
+
+// One bit field of a fused key.
+struct FuseItem {
+    int src = 0;      // key column, or -1: the dense rank of the previous stage's fused key
+    int ranked = 0;   // key column replaced by the dense rank of its range-reduced value
+    int bits = 0;
+    int shift = 0;
+};
+
+// Stage s fuses its items into one 64-bit value per row; the last stage's
+// value is the fused key.  Every earlier stage's value is replaced by its
+// dense rank in the next stage (first item), and `table` maps that rank back.
+struct FuseStage {
+    int nitems = 0;
+    FuseItem items[kMaxFuseItems];
+    DevBuf table;  // distinct fused values of this stage, ascending (all but the last stage)
+};
 
 struct FusedKeys {
     DevBuf fused;                  // rows x uint64 fused keys
     int ncols = 0;
     int64_t rows = 0;
-    int total_bits = 0;
+    int total_bits = 0;            // bits of the fused key
     int32_t type[kMaxFusedCols] = {};
     int bytes[kMaxFusedCols] = {};
-    int bits[kMaxFusedCols] = {};
-    int shift[kMaxFusedCols] = {};
+    int bits[kMaxFusedCols] = {};  // of the range-reduced column
     uint64_t min[kMaxFusedCols] = {};  // of the order-preserving (twiddled) values
+    DevBuf col_table[kMaxFusedCols];   // ranked columns: distinct range-reduced values, ascending
+    std::vector<FuseStage> stages;
 };
 
-// Fuses the key columns; false when they need more than 64 bits (nothing is
-// allocated then).  Synchronises s (one range reduction per column).
-bool fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f);
+// Fuses the key columns.  Columns whose range-reduced widths sum to at most
+// 64 bits take one stage; wider key sets are folded through dense ranks
+// (inputs below 2^32 rows).  Synchronises s (one range reduction per column,
+// one distinct count per rank).
+void fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f);
 
 // The original key columns of `groups` fused keys (device), one buffer per
 // column of the original widths.

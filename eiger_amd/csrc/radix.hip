@@ -97,24 +97,7 @@ __global__ __launch_bounds__(kRB) void radix_hist_kernel(RadixKeys k, int64_t n,
     if ((uint32_t)t <= mask) hist[(int64_t)blockIdx.x * (mask + 1) + t] = h[t];
 }
 
-// exclusive scan of NT values held one per thread (NT threads; wsum: NT/64 words)
-template <int NT>
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum)
-{
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(x, off);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    uint32_t add = 0;
-    for (int i = 0; i < w; ++i) add += wsum[i];
-    __syncthreads();
-    return x - v + add;
-}
+using dev::block_excl_scan;
 
 __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wsum) { return block_excl_scan<256>(v, wsum); }
 

@@ -143,13 +143,84 @@ def test_multikey_errors_and_empty_input():
     # a forced non-hash strategy without key fusion (groupby.cu:287-294)
     with pytest.raises(G.GamaxError):
         run(keys, vals, [(0, A.SUM)], G.groupby_hint(groupby_algo=ALGO.SORT))
-    # two full-range int64 columns need 128 bits
+    # two full-range int64 columns need 128 bits: folded through dense ranks
     wide = [np.array([np.iinfo(np.int64).min, np.iinfo(np.int64).max] * 4, np.int64)] * 2
-    with pytest.raises(G.GamaxError):
-        run(wide, [np.arange(8, dtype=np.int64)], [(0, A.SUM)])
+    out = run(wide, [np.arange(8, dtype=np.int64)], [(0, A.SUM)], G.groupby_hint(groupby_algo=ALGO.SORT,
+                                                                                   enable_skf=True))
+    assert [c.tolist() for c in out] == [[np.iinfo(np.int64).min, np.iinfo(np.int64).max],
+                                         [np.iinfo(np.int64).min, np.iinfo(np.int64).max], [12, 16]]
     empty = run([k[:0] for k in keys], [v[:0] for v in vals], [(0, A.SUM), (-1, A.COUNT_ALL)])
     assert len(empty) == 5 and all(len(c) == 0 for c in empty)
     assert [c.dtype for c in empty[:3]] == [np.int32, np.int64, np.uint8]
+
+
+def wide_case(kind, n, seed):
+    """Key sets wider than 64 bits after range reduction, drawn from a pool of
+    tuples so that groups repeat."""
+    rng = np.random.default_rng(seed)
+    if kind == "two_int64":      # 128 bits: the prefix and the second column both ranked
+        pool = [rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, 700, dtype=np.int64,
+                             endpoint=True) for _ in range(2)]
+        pool[1][:50] = pool[1][0]  # tuples sharing a second column
+    elif kind == "mixed":        # 64 + 32 + 16 + 8 bits of four types
+        pool = [rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, 900, dtype=np.int64, endpoint=True),
+                rng.integers(0, 1 << 32, 900, dtype=np.uint64).astype(np.uint32),
+                rng.integers(-(1 << 15), 1 << 15, 900).astype(np.int16),
+                rng.integers(0, 256, 900).astype(np.uint8)]
+        pool[0][:300] = pool[0][7]  # tuples sharing the first column
+    else:                        # eight full-range uint64 columns: a chain of ranked stages
+        pool = [rng.integers(0, np.iinfo(np.uint64).max, 400, dtype=np.uint64, endpoint=True) for _ in range(8)]
+        for c in range(1, 8):
+            pool[c][:100] = pool[c][0]
+    pick = rng.integers(0, len(pool[0]), n)
+    keys = [p[pick] for p in pool]
+    vals = [rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64), rng.random(n).astype(np.float32)]
+    return keys, vals
+
+
+@pytest.mark.parametrize("algo,skf", [(ALGO.AUTO_ALGO, False), (ALGO.HASH, False), (ALGO.SORT, True),
+                                      (ALGO.SORT_GFUR, True), (ALGO.PARTITION, True)])
+@pytest.mark.parametrize("kind", ["two_int64", "mixed", "eight_uint64"])
+def test_wide_multikey_against_pandas(kind, algo, skf):
+    """Keys needing more than 64 bits (the reference's multi-column hash table,
+    groupby_hash.cu:212-270): per-group results against pandas, key columns
+    decoded in their own types, sort output in ascending tuple order."""
+    n = 1 << 17
+    keys, vals = wide_case(kind, n, seed=len(kind) * 31 + int(algo))
+    reqs = [(0, A.SUM), (-1, A.COUNT_ALL), (0, A.MIN), (1, A.SUM)]
+    out = run(keys, vals, reqs, G.groupby_hint(groupby_algo=algo, enable_skf=skf))
+    nk = len(keys)
+    for j in range(nk):
+        assert out[j].dtype == keys[j].dtype
+    got = list(zip(*[c.tolist() for c in out[:nk]]))
+    if algo in (ALGO.SORT, ALGO.SORT_GFUR):
+        assert got == sorted(got)
+    names = [f"k{j}" for j in range(nk)]
+    df = pd.DataFrame({**{names[j]: keys[j] for j in range(nk)}, "v": vals[0], "f": vals[1].astype(np.float64)})
+    g = df.groupby(names, sort=True)
+    exp = pd.DataFrame({"sum": g["v"].sum(), "count": g["v"].count(), "min": g["v"].min(),
+                        "fsum": g["f"].sum()}).reset_index()
+    order = np.lexsort(tuple(out[j] for j in reversed(range(nk))))
+    out = [c[order] for c in out]
+    assert len(out[0]) == len(exp)
+    for j in range(nk):
+        np.testing.assert_array_equal(out[j], exp[names[j]].to_numpy())
+    np.testing.assert_array_equal(out[nk], exp["sum"].to_numpy())
+    np.testing.assert_array_equal(out[nk + 1], exp["count"].to_numpy())
+    np.testing.assert_array_equal(out[nk + 2], exp["min"].to_numpy())
+    np.testing.assert_allclose(out[nk + 3], exp["fsum"].to_numpy(), rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("fn", [G.hash_distinct, G.sort_distinct])
+def test_wide_multikey_distinct(fn):
+    keys, _ = wide_case("mixed", 1 << 16, seed=5)
+    out = fn([G.make_column_from_host(k) for k in keys], G.groupby_hint(enable_skf=True)).to_numpy()
+    assert [c.dtype for c in out] == [k.dtype for k in keys]
+    exp = sorted(set(zip(*[k.tolist() for k in keys])))
+    if fn is G.sort_distinct:
+        assert list(zip(*[c.tolist() for c in out])) == exp
+    else:
+        assert _rows(out) == exp
 
 
 def _rows(out):
