@@ -10,8 +10,9 @@
 
 namespace eg {
 
-constexpr int kMaxFusedCols = 8;
-constexpr int kMaxFuseItems = kMaxFusedCols + 1;
+// Bit fields one fusion pass writes; a stage that holds this many is folded
+// into its dense rank before the next column.
+constexpr int kMaxFuseItems = 9;
 
 // One bit field of a fused key.
 struct FuseItem {
@@ -35,18 +36,19 @@ struct FusedKeys {
     int ncols = 0;
     int64_t rows = 0;
     int total_bits = 0;            // bits of the fused key
-    int32_t type[kMaxFusedCols] = {};
-    int bytes[kMaxFusedCols] = {};
-    int bits[kMaxFusedCols] = {};  // of the range-reduced column
-    uint64_t min[kMaxFusedCols] = {};  // of the order-preserving (twiddled) values
-    DevBuf col_table[kMaxFusedCols];   // ranked columns: distinct range-reduced values, ascending
+    // per key column
+    std::vector<int32_t> type;
+    std::vector<int> bytes;
+    std::vector<int> bits;          // of the range-reduced column
+    std::vector<uint64_t> min;      // of the order-preserving (twiddled) values
+    std::vector<DevBuf> col_table;  // ranked columns: distinct range-reduced values, ascending
     std::vector<FuseStage> stages;
 };
 
-// Fuses the key columns.  Columns whose range-reduced widths sum to at most
-// 64 bits take one stage; wider key sets are folded through dense ranks
-// (inputs below 2^32 rows).  Synchronises s (one range reduction per column,
-// one distinct count per rank).
+// Fuses the key columns.  Up to nine columns whose range-reduced widths sum
+// to at most 64 bits take one stage; wider or longer key sets are folded
+// through dense ranks (inputs below 2^32 rows).  Synchronises s (one range
+// reduction per column, one distinct count per rank).
 void fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f);
 
 // The original key columns of `groups` fused keys (device), one buffer per

@@ -12,9 +12,10 @@
 // the key-ordered sort output) apply unchanged.  Output keys are decoded from
 // the fused key (unfuse).
 //
-// Key sets needing more than 64 bits (the reference's multi-column hash
-// table, groupby_hash.cu:212-270) are folded: when the next column does not
-// fit beside the columns fused so far, their fused value is replaced by its
+// Key sets needing more than 64 bits or more than nine fields (the
+// reference's multi-column hash table, groupby_hash.cu:212-270) are folded:
+// when the next column does not fit beside the columns fused so far (bits,
+// or the nine fields of one pass), their fused value is replaced by its
 // dense rank (the number of distinct smaller values: a radix sort with row
 // ids, head flags and a scan), which needs at most 32 bits below 2^32 rows;
 // a column that still does not fit is replaced by its own dense rank.  Ranks
@@ -149,6 +150,13 @@ int64_t grid_of(int64_t n)
 // Returns the distinct count.  Synchronises s.
 int64_t dense_ranks(const uint64_t* v, int64_t n, int width, hipStream_t s, DevBuf& ranks, DevBuf& table)
 {
+    if (width == 0) {  // every value is 0: one rank
+        ranks = DevBuf((size_t)n * 4, s);
+        table = DevBuf(8, s);
+        fill_zero(ranks.get(), (size_t)n * 4, s);
+        fill_zero(table.get(), 8, s);
+        return 1;
+    }
     RadixKeys rk{};
     rk.data = v;
     rk.bytes = 8;
@@ -190,10 +198,16 @@ uint64_t field_mask(int bits) { return bits >= 64 ? ~0ull : ((1ull << bits) - 1u
 
 void fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f)
 {
-    if (ncols < 1 || ncols > kMaxFusedCols) raise(EG_ERR_UNSUPPORTED, "at most 8 key columns can be fused");
+    if (ncols < 1 || ncols > (int64_t)1 << 20) raise(EG_ERR_INVALID, "bad key column count");
     const int64_t n = keys[0].size;
     f.ncols = (int)ncols;
     f.rows = n;
+    f.type.assign((size_t)ncols, 0);
+    f.bytes.assign((size_t)ncols, 0);
+    f.bits.assign((size_t)ncols, 0);
+    f.min.assign((size_t)ncols, 0);
+    f.col_table.clear();
+    f.col_table.resize((size_t)ncols);
     for (int64_t c = 0; c < ncols; ++c) {  // all checks before any device work
         if (keys[c].size != n) raise(EG_ERR_INVALID, "key columns have different row counts");
         if (!is_int(keys[c].type) && keys[c].type != EG_BOOL8)
@@ -210,11 +224,11 @@ void fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f)
         f.bits[c] = bit_width(mx - mn);
         total += f.bits[c];
     }
-    if (total > 64 && n >= ((int64_t)1 << 32))
+    if ((total > 64 || ncols > kMaxFuseItems) && n >= ((int64_t)1 << 32))
         raise(EG_ERR_UNSUPPORTED, "multi-column keys wider than 64 bits take fewer than 2^32 rows");
 
     DevBuf prev_ranks;                 // dense ranks of the previous stage's fused values
-    DevBuf col_ranks[kMaxFusedCols];   // dense ranks of ranked columns
+    std::vector<DevBuf> col_ranks((size_t)ncols);  // dense ranks of ranked columns
     // Assigns the stage's shifts (first item in the most significant bits,
     // zero-width items none) and fuses its items for every row.
     auto run_stage = [&](FuseStage& st, int width) {
@@ -231,7 +245,7 @@ void fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f)
                 const FuseItem& it = st.items[j];
                 a.shift[j] = it.shift;
                 if (it.src < 0 || it.ranked) {
-                    a.data[j] = it.src < 0 ? prev_ranks.get() : col_ranks[it.src].get();
+                    a.data[j] = it.src < 0 ? prev_ranks.get() : col_ranks[(size_t)it.src].get();
                     a.bytes[j] = 4;
                     a.type[j] = EG_UINT32;
                     a.min[j] = 0;
@@ -255,7 +269,7 @@ void fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f)
         FuseItem it;
         it.src = c;
         it.bits = f.bits[c];
-        if (width + it.bits > 64) {
+        if (width + it.bits > 64 || cur.nitems == kMaxFuseItems) {
             // the columns so far become the dense rank of their fused value
             DevBuf v = run_stage(cur, width);
             const int64_t nd = dense_ranks(v.get<uint64_t>(), n, width, s, prev_ranks, cur.table);
@@ -270,7 +284,8 @@ void fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f)
                 FuseStage one;
                 one.items[one.nitems++] = it;
                 DevBuf w = run_stage(one, it.bits);
-                const int64_t ndc = dense_ranks(w.get<uint64_t>(), n, it.bits, s, col_ranks[c], f.col_table[c]);
+                const int64_t ndc =
+                    dense_ranks(w.get<uint64_t>(), n, it.bits, s, col_ranks[(size_t)c], f.col_table[(size_t)c]);
                 it.ranked = 1;
                 it.bits = bit_width((uint64_t)(ndc - 1));
             }
@@ -313,7 +328,7 @@ std::vector<DevBuf> unfuse_keys(const FusedKeys& f, const void* fused, int64_t g
                 a.bytes[j] = f.bytes[it.src];
                 a.type[j] = f.type[it.src];
                 a.min[j] = f.min[it.src];
-                a.table[j] = it.ranked ? f.col_table[it.src].get<uint64_t>() : nullptr;
+                a.table[j] = it.ranked ? f.col_table[(size_t)it.src].get<uint64_t>() : nullptr;
             }
         }
         hipLaunchKernelGGL(unfuse_keys_kernel, dim3((unsigned)grid_of(groups)), dim3(256), 0, s, a, groups,

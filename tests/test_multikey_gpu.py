@@ -62,8 +62,19 @@ FIXTURES = [
 ]
 
 
+def _key_col(k):
+    # numpy bools travel as BOOL8 columns (one byte, read back as uint8)
+    if k.dtype == np.bool_:
+        return G.make_column_from_host(k.astype(np.uint8), type=G.type_id.BOOL8)
+    return G.make_column_from_host(k)
+
+
+def _out_dtype(k):
+    return np.dtype(np.uint8) if k.dtype == np.bool_ else k.dtype
+
+
 def run(keys, vals, reqs, hint=None, fn=G.groupby):
-    kcs = [G.make_column_from_host(k) for k in keys]
+    kcs = [_key_col(k) for k in keys]
     vcs = [G.make_column_from_host(v) for v in vals]
     t = fn(kcs, vcs, reqs, hint or G.groupby_hint())
     out = t.to_numpy()
@@ -168,6 +179,17 @@ def wide_case(kind, n, seed):
                 rng.integers(-(1 << 15), 1 << 15, 900).astype(np.int16),
                 rng.integers(0, 256, 900).astype(np.uint8)]
         pool[0][:300] = pool[0][7]  # tuples sharing the first column
+    elif kind == "twelve_narrow":  # more fields than one fusion pass holds, three of them constant
+        pool = []
+        for c in range(12):
+            t = [np.uint8, np.int16, np.bool_, np.int32][c % 4]
+            if c in (2, 7, 10):
+                col = np.full(600, 1 if t is np.bool_ else -3 if t in (np.int16, np.int32) else 5, t)
+            elif t is np.bool_:
+                col = rng.integers(0, 2, 600).astype(np.bool_)
+            else:
+                col = rng.integers(-2 if t in (np.int16, np.int32) else 0, 3, 600).astype(t)
+            pool.append(col)
     else:                        # eight full-range uint64 columns: a chain of ranked stages
         pool = [rng.integers(0, np.iinfo(np.uint64).max, 400, dtype=np.uint64, endpoint=True) for _ in range(8)]
         for c in range(1, 8):
@@ -180,7 +202,7 @@ def wide_case(kind, n, seed):
 
 @pytest.mark.parametrize("algo,skf", [(ALGO.AUTO_ALGO, False), (ALGO.HASH, False), (ALGO.SORT, True),
                                       (ALGO.SORT_GFUR, True), (ALGO.PARTITION, True)])
-@pytest.mark.parametrize("kind", ["two_int64", "mixed", "eight_uint64"])
+@pytest.mark.parametrize("kind", ["two_int64", "mixed", "eight_uint64", "twelve_narrow"])
 def test_wide_multikey_against_pandas(kind, algo, skf):
     """Keys needing more than 64 bits (the reference's multi-column hash table,
     groupby_hash.cu:212-270): per-group results against pandas, key columns
@@ -191,7 +213,7 @@ def test_wide_multikey_against_pandas(kind, algo, skf):
     out = run(keys, vals, reqs, G.groupby_hint(groupby_algo=algo, enable_skf=skf))
     nk = len(keys)
     for j in range(nk):
-        assert out[j].dtype == keys[j].dtype
+        assert out[j].dtype == _out_dtype(keys[j])
     got = list(zip(*[c.tolist() for c in out[:nk]]))
     if algo in (ALGO.SORT, ALGO.SORT_GFUR):
         assert got == sorted(got)
@@ -204,18 +226,33 @@ def test_wide_multikey_against_pandas(kind, algo, skf):
     out = [c[order] for c in out]
     assert len(out[0]) == len(exp)
     for j in range(nk):
-        np.testing.assert_array_equal(out[j], exp[names[j]].to_numpy())
+        np.testing.assert_array_equal(out[j], exp[names[j]].to_numpy().astype(_out_dtype(keys[j])))
     np.testing.assert_array_equal(out[nk], exp["sum"].to_numpy())
     np.testing.assert_array_equal(out[nk + 1], exp["count"].to_numpy())
     np.testing.assert_array_equal(out[nk + 2], exp["min"].to_numpy())
     np.testing.assert_allclose(out[nk + 3], exp["fsum"].to_numpy(), rtol=1e-6, atol=0)
 
 
+@pytest.mark.parametrize("algo,skf", [(ALGO.HASH, False), (ALGO.SORT, True), (ALGO.PARTITION, True)])
+def test_ten_constant_key_columns(algo, skf):
+    """Ten constant columns: a full fusion pass of zero-width fields folds
+    into a zero-width rank; one group holding every row."""
+    n = 5000
+    types = [np.int8, np.uint16, np.int32, np.int64, np.uint8, np.bool_, np.uint32, np.int16, np.uint64, np.int64]
+    keys = [np.full(n, 1 if t is np.bool_ else 7 * (i + 1), t) for i, t in enumerate(types)]
+    vals = [np.arange(n, dtype=np.int64)]
+    out = run(keys, vals, [(0, A.SUM), (-1, A.COUNT_ALL)], G.groupby_hint(groupby_algo=algo, enable_skf=skf))
+    assert len(out) == 12 and all(len(c) == 1 for c in out)
+    for j, k in enumerate(keys):
+        assert out[j].dtype == _out_dtype(k) and out[j][0] == k[0]
+    assert out[10][0] == n * (n - 1) // 2 and out[11][0] == n
+
+
 @pytest.mark.parametrize("fn", [G.hash_distinct, G.sort_distinct])
 def test_wide_multikey_distinct(fn):
     keys, _ = wide_case("mixed", 1 << 16, seed=5)
-    out = fn([G.make_column_from_host(k) for k in keys], G.groupby_hint(enable_skf=True)).to_numpy()
-    assert [c.dtype for c in out] == [k.dtype for k in keys]
+    out = fn([_key_col(k) for k in keys], G.groupby_hint(enable_skf=True)).to_numpy()
+    assert [c.dtype for c in out] == [_out_dtype(k) for k in keys]
     exp = sorted(set(zip(*[k.tolist() for k in keys])))
     if fn is G.sort_distinct:
         assert list(zip(*[c.tolist() for c in out])) == exp
