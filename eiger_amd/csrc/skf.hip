@@ -208,6 +208,7 @@ void fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f)
     f.min.assign((size_t)ncols, 0);
     f.col_table.clear();
     f.col_table.resize((size_t)ncols);
+    f.stages.clear();
     for (int64_t c = 0; c < ncols; ++c) {  // all checks before any device work
         if (keys[c].size != n) raise(EG_ERR_INVALID, "key columns have different row counts");
         if (!is_int(keys[c].type) && keys[c].type != EG_BOOL8)
