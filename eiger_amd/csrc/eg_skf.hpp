@@ -37,6 +37,7 @@ struct FusedKeys {
     int64_t rows = 0;
     int total_bits = 0;            // bits of the fused key
     // per key column
+    std::vector<int32_t> orig_type;  // the caller's type (float columns are fused in mapped form)
     std::vector<int32_t> type;
     std::vector<int> bytes;
     std::vector<int> bits;          // of the range-reduced column

@@ -3,7 +3,8 @@
 // compress/multikey_compress.hpp:11-43, used by groupby.cu:396-465).
 //
 // Each integer key column c is mapped to its order-preserving unsigned form
-// t_c (sign bit flipped for signed types), its range [min_c, max_c] is read
+// t_c (sign bit flipped for signed types; float columns first take their
+// mapped integer form, keynorm.hip), its range [min_c, max_c] is read
 // with one reduction, and it takes bits_c = bit width of (max_c - min_c) bits
 // of the fused key, the first column in the most significant bits:
 //     fused = sum_c (t_c - min_c) << shift_c.
@@ -209,18 +210,33 @@ void fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f)
     f.col_table.clear();
     f.col_table.resize((size_t)ncols);
     f.stages.clear();
+    f.orig_type.assign((size_t)ncols, 0);
     for (int64_t c = 0; c < ncols; ++c) {  // all checks before any device work
         if (keys[c].size != n) raise(EG_ERR_INVALID, "key columns have different row counts");
-        if (!is_int(keys[c].type) && keys[c].type != EG_BOOL8)
-            raise(EG_ERR_UNSUPPORTED, "multi-column keys must be integer columns");
+        if (!is_int(keys[c].type) && keys[c].type != EG_BOOL8 && !is_float(keys[c].type))
+            raise(EG_ERR_UNSUPPORTED, "multi-column keys must be integer, BOOL8 or float columns");
     }
+    // float columns are fused through their mapped integer form (-0.0 as
+    // +0.0, every NaN as one value, numeric order; keynorm.hip)
+    std::vector<DevBuf> mapped((size_t)ncols);
+    std::vector<const void*> src((size_t)ncols);
     int total = 0;
     for (int64_t c = 0; c < ncols; ++c) {
         const ViewIn& k = keys[c];
-        f.type[c] = k.type;
-        f.bytes[c] = type_size(k.type);
+        f.orig_type[c] = k.type;
+        src[c] = k.data;
+        int32_t t = k.type;
+        if (is_float(t)) {
+            if (n > 0) {
+                mapped[c] = map_keys(k.data, t, n, s);
+                src[c] = mapped[c].get();
+            }
+            t = mapped_key_type(t);
+        }
+        f.type[c] = t;
+        f.bytes[c] = type_size(t);
         uint64_t mn = 0, mx = 0;
-        if (n > 0) key_range(k.data, k.type, f.bytes[c], n, s, &mn, &mx);
+        if (n > 0) key_range(src[c], t, f.bytes[c], n, s, &mn, &mx);
         f.min[c] = mn;
         f.bits[c] = bit_width(mx - mn);
         total += f.bits[c];
@@ -251,7 +267,7 @@ void fuse_keys(const ViewIn* keys, int64_t ncols, hipStream_t s, FusedKeys& f)
                     a.type[j] = EG_UINT32;
                     a.min[j] = 0;
                 } else {
-                    a.data[j] = keys[it.src].data;
+                    a.data[j] = src[(size_t)it.src];
                     a.bytes[j] = f.bytes[it.src];
                     a.type[j] = f.type[it.src];
                     a.min[j] = f.min[it.src];
@@ -338,6 +354,8 @@ std::vector<DevBuf> unfuse_keys(const FusedKeys& f, const void* fused, int64_t g
         cur_buf = std::move(prev);
         cur = cur_buf.get();
     }
+    for (int c = 0; c < f.ncols; ++c)
+        if (is_float(f.orig_type[(size_t)c])) cols[(size_t)c] = unmap_keys(cols[(size_t)c].get(), groups, f.orig_type[(size_t)c], s);
     return cols;
 }
 

@@ -111,16 +111,16 @@ void eg_hint_init(eg_groupby_hint* hint);
  * groupby.cu:396-465).  Result columns: [key, one column per request in
  * request order, trailing COUNT if hint->agg_count] (groupby.cu:326-388), typed
  * as agg_output_data_type (aggregate.cu:20-53); num_rows = number of groups.
- * Row order is strategy-dependent and unspecified.  Integer keys of 4 or 8
- * bytes; several integer key columns are fused into one 64-bit key
- * after range reduction (the reference's SKF path, groupby.cu:396-465;
+ * Row order is strategy-dependent and unspecified.  A key column may be any
+ * integer type, BOOL8, FLOAT32 or FLOAT64 (partition: integers only).
+ * Several key columns (integer, BOOL8 or float) are fused into one 64-bit
+ * key after range reduction (the reference's SKF path, groupby.cu:396-465;
  * compress/multikey_compress.hpp:11-43) -- with AUTO or HASH, or any
  * strategy when enable_skf is set.  Key sets wider than 64 bits or of more
  * than nine columns (the reference's multi-column hash table,
  * groupby_hash.cu:212-270) are folded through order-preserving dense ranks
- * first (inputs below 2^32 rows).  An
- * empty input gives a 0-row table with that
- * schema (phys/phys.cu:589-611). */
+ * first (inputs below 2^32 rows).  An empty input gives a 0-row table with
+ * that schema (phys/phys.cu:589-611). */
 int eg_groupby(const eg_column_view* keys, int64_t num_keys,
                const eg_column_view* vals, int64_t num_vals,
                const eg_agg_request* reqs, int64_t num_reqs,

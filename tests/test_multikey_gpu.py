@@ -233,6 +233,38 @@ def test_wide_multikey_against_pandas(kind, algo, skf):
     np.testing.assert_allclose(out[nk + 3], exp["fsum"].to_numpy(), rtol=1e-6, atol=0)
 
 
+@pytest.mark.parametrize("algo,skf", [(ALGO.AUTO_ALGO, False), (ALGO.HASH, False), (ALGO.SORT, True),
+                                      (ALGO.PARTITION, True)])
+def test_float_key_columns_against_pandas(algo, skf):
+    """Float key columns among integer ones: -0.0 groups with +0.0 and every
+    NaN (any payload) falls in one group, as pandas' group-by with
+    dropna=False does; sort output in ascending order, NaN last."""
+    rng = np.random.default_rng(int(algo) + 40)
+    n = 1 << 16
+    other_nan = np.array([0x7fc00123], np.uint32).view(np.float32)[0]
+    f32 = np.array([0.0, -0.0, np.nan, other_nan, 1.5, -1.5, np.inf, -np.inf, 3e-39, 7.0], np.float32)
+    f64 = np.array([0.0, -0.0, np.nan, 2.5, -2.5, 1e300, -1e-300, 4.0], np.float64)
+    keys = [f32[rng.integers(0, len(f32), n)], rng.integers(-3, 3, n).astype(np.int32),
+            f64[rng.integers(0, len(f64), n)]]
+    vals = [rng.integers(-1000, 1000, n).astype(np.int64)]
+    out = run(keys, vals, [(0, A.SUM), (-1, A.COUNT_ALL)], G.groupby_hint(groupby_algo=algo, enable_skf=skf))
+    assert [c.dtype for c in out[:3]] == [np.float32, np.int32, np.float64]
+    order = np.lexsort((out[2], out[1], out[0]))
+    if algo == ALGO.SORT:
+        np.testing.assert_array_equal(order, np.arange(len(order)))
+    out = [c[order] for c in out]
+    df = pd.DataFrame({"a": keys[0], "b": keys[1], "c": keys[2], "v": vals[0]})
+    g = df.groupby(["a", "b", "c"], sort=True, dropna=False)["v"]
+    exp = pd.DataFrame({"sum": g.sum(), "count": g.count()}).reset_index()
+    assert len(out[0]) == len(exp)
+    np.testing.assert_array_equal(out[0], exp["a"].to_numpy())
+    np.testing.assert_array_equal(out[1], exp["b"].to_numpy())
+    np.testing.assert_array_equal(out[2], exp["c"].to_numpy())
+    np.testing.assert_array_equal(out[3], exp["sum"].to_numpy())
+    np.testing.assert_array_equal(out[4], exp["count"].to_numpy())
+    assert not np.any(np.signbit(out[0]) & (out[0] == 0))  # the zero group's key is +0.0
+
+
 @pytest.mark.parametrize("algo,skf", [(ALGO.HASH, False), (ALGO.SORT, True), (ALGO.PARTITION, True)])
 def test_ten_constant_key_columns(algo, skf):
     """Ten constant columns: a full fusion pass of zero-width fields folds
