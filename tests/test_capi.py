@@ -59,8 +59,9 @@ def test_no_key_columns_is_invalid():
 def test_multi_column_key_requests_rejected_before_device_work():
     """Multi-column keys go through key fusion; the sort and partition
     strategy entry points, and a forced non-hash strategy without enable_skf,
-    take one key column (groupby_sort.cu:209-211, groupby.cu:287-294).  Non-
-    integer key columns cannot be fused; key columns must have equal lengths."""
+    take one key column (groupby_sort.cu:209-211, groupby.cu:287-294).  A key
+    column without a type cannot be fused; key columns must have equal
+    lengths."""
     two = [_cv(), _cv()]
     for fn in (G.sort_groupby, G.sort_groupby_gfur, G.partition_groupby, G.partition_groupby_gfur):
         with pytest.raises(G.GamaxError) as e:
@@ -71,7 +72,7 @@ def test_multi_column_key_requests_rejected_before_device_work():
             G.groupby(two, [_cv()], [(0, G.aggregation_id.SUM)], G.groupby_hint(groupby_algo=algo))
         assert e.value.status == G.EG_ERR_UNSUPPORTED
     with pytest.raises(G.GamaxError) as e:
-        G.groupby([_cv(), _cv(t=G.type_id.FLOAT32)], [_cv()], [(0, G.aggregation_id.SUM)])
+        G.groupby([_cv(), _cv(t=G.type_id.EMPTY)], [_cv()], [(0, G.aggregation_id.SUM)])
     assert e.value.status == G.EG_ERR_UNSUPPORTED
     with pytest.raises(G.GamaxError) as e:
         G.groupby([_cv(n=10), _cv(n=11)], [_cv()], [(0, G.aggregation_id.SUM)])
