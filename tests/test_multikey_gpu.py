@@ -233,6 +233,34 @@ def test_wide_multikey_against_pandas(kind, algo, skf):
     np.testing.assert_allclose(out[nk + 3], exp["fsum"].to_numpy(), rtol=1e-6, atol=0)
 
 
+@pytest.mark.parametrize("algo,skf", [(ALGO.HASH, False), (ALGO.SORT, True), (ALGO.PARTITION, True)])
+def test_wide_multikey_many_groups(algo, skf):
+    """Two full-range int64 columns, 2^20 rows in about 2^19 groups: the dense
+    ranks run over 256 tiles with many groups in each, so every tile's rank
+    offset comes from the scan across tiles."""
+    rng = np.random.default_rng(77 + int(algo))
+    n = 1 << 20
+    pool = [rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, 1 << 19, dtype=np.int64, endpoint=True)
+            for _ in range(2)]
+    pool[0][: 1 << 16] = pool[0][0]  # a first column shared by 2^16 tuples
+    pick = rng.integers(0, 1 << 19, n)
+    keys = [p[pick] for p in pool]
+    vals = [rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)]
+    out = run(keys, vals, [(0, A.SUM), (-1, A.COUNT_ALL)], G.groupby_hint(groupby_algo=algo, enable_skf=skf))
+    order = np.lexsort((out[1], out[0]))
+    if algo == ALGO.SORT:
+        np.testing.assert_array_equal(order, np.arange(len(order)))
+    out = [c[order] for c in out]
+    df = pd.DataFrame({"a": keys[0], "b": keys[1], "v": vals[0]})
+    g = df.groupby(["a", "b"], sort=True)["v"]
+    exp = pd.DataFrame({"sum": g.sum(), "count": g.count()}).reset_index()
+    assert len(out[0]) == len(exp)
+    np.testing.assert_array_equal(out[0], exp["a"].to_numpy())
+    np.testing.assert_array_equal(out[1], exp["b"].to_numpy())
+    np.testing.assert_array_equal(out[2], exp["sum"].to_numpy())
+    np.testing.assert_array_equal(out[3], exp["count"].to_numpy())
+
+
 @pytest.mark.parametrize("algo,skf", [(ALGO.AUTO_ALGO, False), (ALGO.HASH, False), (ALGO.SORT, True),
                                       (ALGO.PARTITION, True)])
 def test_float_key_columns_against_pandas(algo, skf):
